@@ -1,0 +1,112 @@
+/*
+ * toporecon_hip: C ABI of the MI355X-native PDHG reconstruction hot path.
+ *
+ * All pointers are device pointers unless stated otherwise; complex values are
+ * interleaved float32 pairs (complex64).  Every entry point returns 0 on
+ * success or a negative status (-1 invalid argument, -2 HIP error,
+ * -3 unsupported, -4 out of memory) and leaves a message for tr_last_error().
+ * `stream` is a hipStream_t (NULL = legacy default stream).  A plan is not
+ * thread-safe: calls on one plan must be serialised on one stream, because
+ * they share the plan's grid workspace.
+ *
+ * Layouts: images (B, E, Z, Y, X) / maps (C, Z, Y, X) C-order; per-bin k-space
+ * (E, C, M_b) = (K, M_b) with channel k = e * C + c; "sorted" per-sample data
+ * (M_b, K) in the plan's tile order of the bin's samples.
+ */
+#ifndef TOPORECON_HIP_H_
+#define TOPORECON_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TR_ABI_VERSION 1
+
+typedef struct tr_plan tr_plan;
+
+/* complex64 as stored by NumPy / PyTorch: real then imaginary float32. */
+typedef struct tr_c64 {
+  float re;
+  float im;
+} tr_c64;
+
+/* Message of the last failing call on this thread.
+ * Replaces the Python exceptions of src/toporecon/nufft_backend.py:19-22,46-52,80-87. */
+const char* tr_last_error(void);
+int tr_version(void);
+
+/* ---- NUFFT plan (SigPy 0.1.27 nufft defaults: width 4, oversampling 1.25) ----
+ * Replaces SigPyNufftBackend.__init__ (src/toporecon/nufft_backend.py:58-88):
+ * coords[b] is a device (M_b, 3) float32 array in [z, y, x] SigPy units.  The
+ * plan scales the coordinates like SigPy's _scale_coord, sorts each bin's
+ * samples by grid tile and builds the deterministic spreading CSR. */
+int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
+                   const int64_t* samples_per_bin, void* stream);
+int tr_plan_destroy(tr_plan* plan);
+/* info[0..2] image dims, [3..5] grid dims, [6] bins, [7] M_b, [8] nnz_b, [9] workspace channels. */
+int tr_plan_info(const tr_plan* plan, int bin, int64_t* info);
+int tr_plan_release_workspace(tr_plan* plan);
+
+/* Sample-order helpers for the fused solver path (no reference counterpart:
+ * the reference keeps samples in trajectory order).  gather: (K, M_b) original
+ * order -> (M_b, K) sorted; scatter: inverse; gather_weights: (M_b) -> (M_b). */
+int tr_plan_gather_samples(const tr_plan* plan, int bin, const tr_c64* src, int K, tr_c64* dst,
+                           void* stream);
+int tr_plan_scatter_samples(const tr_plan* plan, int bin, const tr_c64* src, int K, tr_c64* dst,
+                            void* stream);
+int tr_plan_gather_weights(const tr_plan* plan, int bin, const float* src, float* dst, void* stream);
+
+/* Forward NUFFT of E*C images of one bin: out[(e*C + c), m] = nufft(img[e] * maps[c]).
+ * Replaces SigPyNufftBackend.forward -> sp.nufft (src/toporecon/nufft_backend.py:101-107);
+ * maps == NULL requires C == 1.  out is (E*C, M_b) in the trajectory's order. */
+int tr_nufft_forward(tr_plan* plan, int bin, const tr_c64* img, const tr_c64* maps, int E, int C,
+                     tr_c64* out, void* stream);
+
+/* Adjoint NUFFT: out[e] (+)= sum_c conj(maps[c]) * nufft_adjoint(dcf * ksp[e*C + c]).
+ * Replaces SigPyNufftBackend.adjoint -> sp.nufft_adjoint (src/toporecon/nufft_backend.py:109-116)
+ * together with the coil loop of tvm.py:216-225; dcf may be NULL, maps == NULL requires C == 1. */
+int tr_nufft_adjoint(tr_plan* plan, int bin, const tr_c64* ksp, const float* dcf, const tr_c64* maps,
+                     int E, int C, tr_c64* out, int accumulate, void* stream);
+
+/* Fused data-term dual step of one bin on sorted samples:
+ * q <- (q + sigma (A ubar_b - y)) / one_plus_sigma.
+ * Replaces the forward triple loop + prox of tvm.py:246-260 / tvme.py:257-267 / tvmw.py:454-465. */
+int tr_data_dual_update(tr_plan* plan, int bin, const tr_c64* ubar_b, const tr_c64* maps, int E, int C,
+                        const tr_c64* y_sorted, tr_c64* q_sorted, float sigma, float one_plus_sigma,
+                        void* stream);
+/* y_sorted = A x_b (synthetic k-space; no reference counterpart). */
+int tr_data_simulate(tr_plan* plan, int bin, const tr_c64* x_b, const tr_c64* maps, int E, int C,
+                     tr_c64* y_sorted, void* stream);
+/* g_b (+)= A^H W q: sum_c conj(S_c) NUFFT^H(dcf * q_c) on sorted samples.
+ * Replaces tvm.py:216-225 (+ coil all-reduce tvm.py:188-214), tvme.py:383-419, tvmw.py:586-629. */
+int tr_data_adjoint(tr_plan* plan, int bin, const tr_c64* q_sorted, const float* dcf_sorted,
+                    const tr_c64* maps, int E, int C, tr_c64* g_b, int accumulate, void* stream);
+
+/* ---- element-wise PDHG steps on (B, E, Z, Y, X) arrays ----
+ * `partial` holds tr_partial_count() float64 pairs (sum |u - u_old|^2, sum |u|^2)
+ * reduced by tr_reduce_partials into out2[2]. */
+int64_t tr_partial_count(int B, int Z, int Y, int X);
+int tr_reduce_partials(const double* partial, int64_t n, double* out2, void* stream);
+
+/* TVM motion dual + prox (tvm.py:262-309). */
+int tr_tvm_dual(int B, int E, int Z, int Y, int X, const tr_c64* ubar, tr_c64* p_m, float sigma,
+                float lam, int l2_coupling, void* stream);
+/* TVM primal + extrapolation + stopping sums (tvm.py:342-379, 416-424). */
+int tr_tvm_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, const tr_c64* grad,
+                  const tr_c64* p_m, float tau, double* partial, void* stream);
+/* TVME motion + echo/spatial TV duals and prox (tvme.py:233-337). */
+int tr_tvme_dual(int B, int E, int Z, int Y, int X, const tr_c64* ubar, tr_c64* p_m, tr_c64* p_ex,
+                 tr_c64* p_ey, tr_c64* p_ez, float sigma, float lam_m, float lam_e, int l2_coupling,
+                 void* stream);
+/* TVME divergences, primal update, extrapolation, stopping sums (tvme.py:356-480, 520-523). */
+int tr_tvme_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, const tr_c64* grad,
+                   const tr_c64* p_m, const tr_c64* p_ex, const tr_c64* p_ey, const tr_c64* p_ez,
+                   float tau, double* partial, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPORECON_HIP_H_ */

@@ -1,0 +1,217 @@
+// LDS-resident mixed-radix Stockham FFT along one axis of a channel-innermost grid.
+//
+// A workgroup holds one line of L points for KC channels in LDS as
+// buf[pos * ld + ch] (ld = KC + 1 for KC >= 4 so the row stride is odd in
+// 8-byte words).  Lane ch of every worker owns one channel, so every LDS access
+// of a stage is conflict-free and every global access is a run of KC
+// consecutive complex values.  Stages run the autosort (Stockham) recursion:
+// butterfly j of a radix-R stage reads x[j + r L/R], twiddles by
+// w_L^{(j mod Ns) r L/(Ns R)}, applies DFT_R and writes
+// y[(j / Ns) Ns R + (j mod Ns) + r Ns].  Each worker keeps its butterflies of a
+// stage in registers across one barrier, so a single LDS buffer suffices.
+// The inverse transform is conj(FFT(conj(x))): callers conjugate on load and
+// store.
+#pragma once
+
+#include "tr_common.hpp"
+
+namespace tr {
+
+constexpr int kBlock = 256;      // threads per FFT workgroup
+constexpr int kMaxFftLen = 256;  // longest supported axis (oversampled grid)
+constexpr int kMaxStages = 12;
+
+struct FftAxis {
+  int L;
+  int nstages;
+  int radix[kMaxStages];
+};
+
+__host__ __device__ constexpr int lds_ld(int kc) { return kc >= 4 ? kc + 1 : kc; }
+
+// ---- small DFT kernels (forward, w = exp(-2 pi i / R)) --------------------
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }  // -i * a
+
+template <int R>
+struct Dft;
+
+template <>
+struct Dft<2> {
+  __device__ static __forceinline__ void run(float2 (&v)[2], const float2*, int) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <>
+struct Dft<3> {
+  __device__ static __forceinline__ void run(float2 (&v)[3], const float2*, int) {
+    const float s = 0.86602540378443864676f;  // sin(2 pi / 3)
+    const float2 t1 = cadd(v[1], v[2]);
+    const float2 d = csub(v[1], v[2]);
+    const float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    v[0] = cadd(v[0], t1);
+    v[1] = make_float2(t2.x + s * d.y, t2.y - s * d.x);
+    v[2] = make_float2(t2.x - s * d.y, t2.y + s * d.x);
+  }
+};
+
+template <>
+struct Dft<4> {
+  __device__ static __forceinline__ void run(float2 (&v)[4], const float2*, int) {
+    const float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    const float2 c = cadd(v[1], v[3]), d = mul_mi(csub(v[1], v[3]));
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+  }
+};
+
+template <>
+struct Dft<5> {
+  __device__ static __forceinline__ void run(float2 (&v)[5], const float2*, int) {
+    const float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
+    const float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
+    const float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
+    const float s2 = 0.58778525229247312917f;   // sin(4 pi / 5)
+    const float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    const float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    const float2 x0 = v[0];
+    const float2 r1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    const float2 r2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    const float2 i1 = make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+    const float2 i2 = make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+    v[0] = make_float2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+    v[1] = cadd(r1, mul_mi(i1));
+    v[4] = csub(r1, mul_mi(i1));
+    v[2] = cadd(r2, mul_mi(i2));
+    v[3] = csub(r2, mul_mi(i2));
+  }
+};
+
+template <>
+struct Dft<8> {
+  __device__ static __forceinline__ void run(float2 (&v)[8], const float2* tw, int L) {
+    const float h = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4>::run(e, tw, L);
+    Dft<4>::run(o, tw, L);
+    // o[k] *= w8^k, w8 = (1 - i)/sqrt(2)
+    o[1] = make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    o[2] = mul_mi(o[2]);
+    o[3] = make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// Generic odd prime radix via the line's twiddle table: w_R^m = tw[m L / R].
+template <int R>
+struct Dft {
+  __device__ static __forceinline__ void run(float2 (&v)[R], const float2* tw, int L) {
+    const int step = L / R;
+    float2 out[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const float2 w = tw[((r * k) % R) * step];
+        acc = cadd(acc, cmul(v[r], w));
+      }
+      out[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+  }
+};
+
+// One radix-R stage over the line held in buf.  W workers per channel.
+template <int R, int W>
+__device__ __forceinline__ void fft_stage(float2* buf, int ld, int ch, int w, int L, int Ns,
+                                          const float2* tw) {
+  constexpr int NB = (kMaxFftLen / R + W - 1) / W;
+  const int nb = L / R;
+  const int tstride = L / (Ns * R);
+  float2 v[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = w + b * W;
+    if (j < nb) {
+      const int jm = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 a = buf[(j + r * nb) * ld + ch];
+        if (r > 0 && jm != 0) a = cmul(a, tw[jm * r * tstride]);
+        v[b][r] = a;
+      }
+      Dft<R>::run(v[b], tw, L);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = w + b * W;
+    if (j < nb) {
+      const int jm = j % Ns;
+      const int base = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * ld + ch] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full forward FFT of the line in buf (natural order in, natural order out).
+// Must be called by every thread of the block (contains barriers).
+template <int W>
+__device__ __forceinline__ void fft_line(float2* buf, int ld, int ch, int w, const FftAxis& ax,
+                                         const float2* tw) {
+  int Ns = 1;
+  for (int s = 0; s < ax.nstages; ++s) {
+    const int R = ax.radix[s];
+    switch (R) {
+      case 2: fft_stage<2, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 3: fft_stage<3, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 4: fft_stage<4, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 5: fft_stage<5, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 7: fft_stage<7, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 8: fft_stage<8, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 11: fft_stage<11, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 13: fft_stage<13, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      default: break;  // rejected on the host
+    }
+    Ns *= R;
+  }
+}
+
+// Host: factor L into supported radices (8s first, then 4, 2, odd primes).
+inline bool make_fft_axis(int L, FftAxis* ax) {
+  if (L < 1 || L > kMaxFftLen) return false;
+  ax->L = L;
+  ax->nstages = 0;
+  int n = L;
+  auto take = [&](int r) {
+    while (n % r == 0 && ax->nstages < kMaxStages) {
+      ax->radix[ax->nstages++] = r;
+      n /= r;
+    }
+  };
+  take(8);
+  take(4);
+  take(2);
+  take(3);
+  take(5);
+  take(7);
+  take(11);
+  take(13);
+  return n == 1;
+}
+
+}  // namespace tr

@@ -1,0 +1,105 @@
+"""ctypes binding of the toporecon HIP C ABI (include/toporecon_hip.h).
+
+The shared library is built in-tree (``multinodemri-miccai26_amd/build.py``)
+as ``toporecon_hip/libtoporecon_hip.so``.  There is no CPU fallback: when the
+library is missing or fails to load, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtoporecon_hip.so")
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_f32 = ctypes.c_float
+c_vp = ctypes.c_void_p
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "tr_last_error": (ctypes.c_char_p, []),
+    "tr_version": (c_int, []),
+    "tr_plan_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int,
+                               ctypes.POINTER(c_vp), ctypes.POINTER(c_i64), c_vp]),
+    "tr_plan_destroy": (c_int, [c_vp]),
+    "tr_plan_info": (c_int, [c_vp, c_int, ctypes.POINTER(c_i64)]),
+    "tr_plan_release_workspace": (c_int, [c_vp]),
+    "tr_plan_gather_samples": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp]),
+    "tr_plan_scatter_samples": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp]),
+    "tr_plan_gather_weights": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
+    "tr_nufft_forward": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
+    "tr_nufft_adjoint": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_int, c_vp]),
+    "tr_data_dual_update": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_f32, c_f32,
+                                    c_vp]),
+    "tr_data_simulate": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
+    "tr_data_adjoint": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_int, c_vp]),
+    "tr_partial_count": (c_i64, [c_int, c_int, c_int, c_int]),
+    "tr_reduce_partials": (c_int, [c_vp, c_i64, c_vp, c_vp]),
+    "tr_tvm_dual": (c_int, [c_int] * 5 + [c_vp, c_vp, c_f32, c_f32, c_int, c_vp]),
+    "tr_tvm_primal": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp, c_vp, c_f32, c_vp, c_vp]),
+    "tr_tvme_dual": (c_int, [c_int] * 5 + [c_vp] * 5 + [c_f32, c_f32, c_f32, c_int, c_vp]),
+    "tr_tvme_primal": (c_int, [c_int] * 5 + [c_vp] * 7 + [c_f32, c_vp, c_vp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class HipLibraryError(RuntimeError):
+    """The native library is missing, failed to load, or a call failed."""
+
+
+def load():
+    """Load (once) and return the ctypes library object."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise HipLibraryError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python multinodemri-miccai26_amd/build.py` (there is no CPU fallback).")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def exported_symbols():
+    return list(SIGNATURES)
+
+
+def last_error() -> str:
+    msg = load().tr_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        raise HipLibraryError(f"{what or 'toporecon_hip call'} failed ({rc}): {last_error()}")
+
+
+def call(name: str, *args):
+    rc = getattr(load(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a tensor (None for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,220 @@
+"""Shared single-GPU machinery of the HIP PDHG reconstructors.
+
+Mirrors the constructor work common to TvmReconstructor (tvm.py:66-178),
+TvmeReconstructor (tvme.py:57-149) and TvmwReconstructor (tvmw.py:214-310):
+motion binning with ``edge_b <= r < edge_{b+1}`` (tvm.py:147-168), per-bin
+k-space / trajectory / DCF / dual slices, and the NUFFT plan.  On the device,
+each bin's k-space ``y`` and data dual ``q`` live as (M_b, E*C) complex64 in the
+plan's sorted sample order, so one launch serves every echo and coil.
+
+One process drives one GPU: the MPI / NCCL arguments of the reference are
+accepted for signature compatibility, but a communicator with more than one
+rank raises ``NotImplementedError``.
+"""
+
+from __future__ import annotations
+
+import logging
+import math
+import time
+
+import numpy as np
+
+from .. import _lib
+from ..nufft_backend import NufftPlan, resolve_device
+
+
+def _comm_size(comm):
+    if comm is None:
+        return 1
+    for attr in ("Get_size", "size"):
+        v = getattr(comm, attr, None)
+        if v is not None:
+            return int(v() if callable(v) else v)
+    return 1
+
+
+class HipPdhgBase:
+    """Binned data, plan and run loop shared by the TVM / TVME / TVMW HIP solvers."""
+
+    algorithm = "base"
+
+    def _setup(self, ksp, coord, dcf, mps, resp, dual_q, B_local, *, margin, device, B_total, b0,
+               b1, bin_edges, comms, show_pbar, nufft_backend, sigma, tau, max_iter, tol):
+        import torch
+        if str(nufft_backend).strip().lower() != "hip":
+            raise ValueError(f"the HIP reconstructors run the 'hip' NUFFT backend, got {nufft_backend!r}")
+        for c in comms:
+            if _comm_size(c) > 1:
+                raise NotImplementedError("the HIP reconstructor drives one GPU per process; "
+                                          "multi-rank communicators are not supported")
+        self.device = resolve_device(device)
+        self.B = int(B_local)
+        self.B_total = int(B_total) if B_total is not None else self.B
+        self.b0 = int(b0) if b0 is not None else 0
+        self.b1 = int(b1) if b1 is not None else self.b0 + self.B
+        if self.b0 != 0 or self.b1 != self.B_total or self.b1 - self.b0 != self.B:
+            raise NotImplementedError("motion-axis sharding needs halo exchanges; one GPU owns all bins")
+        self.sigma = sigma
+        self.tau = tau
+        self.max_iter = int(max_iter)
+        self.tol = tol
+        self.show_pbar = show_pbar
+        self.C = int(mps.shape[0])
+        self.E = int(ksp.shape[0])
+        self.K = self.E * self.C
+        self.img_shape = [int(v) for v in mps.shape[-3:]]
+
+        resp = np.asarray(resp if not isinstance(resp, torch.Tensor) else resp.cpu().numpy()).reshape(-1)
+        if bin_edges is None:
+            bins = np.percentile(resp, np.linspace(0 + margin, 100 - margin, self.B_total + 1))
+        else:
+            bins = np.asarray(bin_edges)
+            assert bins.ndim == 1 and bins.size == self.B_total + 1, \
+                f"bin_edges must have shape ({self.B_total + 1},), got {bins.shape}"
+        self.bin_edges = bins
+
+        dev = self.device
+        self.maps = self._tensor(mps, torch.complex64)
+        coords, self.y, self.q, self.dcf = [], [], [], []
+        members = []
+        for gb in range(self.b0, self.b1):
+            idx = np.nonzero((resp >= bins[gb]) & (resp < bins[gb + 1]))[0]
+            members.append(idx)
+            c = self._take(coord, idx, axis=0, dtype=torch.float32).reshape(-1, 3)
+            coords.append(c)
+        self.members = members
+        self.plan = NufftPlan(self.img_shape, coords, dev)
+        stream = _lib.stream_handle(dev)
+        for b, idx in enumerate(members):
+            M = self.plan.samples(b)
+            k_b = self._take(ksp, idx, axis=2, dtype=torch.complex64).reshape(self.K, M)
+            y_s = torch.empty((M, self.K), dtype=torch.complex64, device=dev)
+            _lib.call("tr_plan_gather_samples", self.plan.handle, b, k_b.data_ptr(), self.K,
+                      y_s.data_ptr(), stream)
+            del k_b
+            q_s = torch.zeros((M, self.K), dtype=torch.complex64, device=dev)
+            if dual_q is not None:
+                q_b = self._take(dual_q, idx, axis=2, dtype=torch.complex64).reshape(self.K, M)
+                _lib.call("tr_plan_gather_samples", self.plan.handle, b, q_b.data_ptr(), self.K,
+                          q_s.data_ptr(), stream)
+                del q_b
+            d_b = self._take(dcf, idx, axis=0, dtype=torch.float32).reshape(M)
+            d_s = torch.empty((M,), dtype=torch.float32, device=dev)
+            _lib.call("tr_plan_gather_weights", self.plan.handle, b, d_b.data_ptr(), d_s.data_ptr(),
+                      stream)
+            self.y.append(y_s)
+            self.q.append(q_s)
+            self.dcf.append(d_s)
+        self.iterations = 0
+        self.elapsed = 0.0
+        self.history = []
+
+    # -- helpers ----------------------------------------------------------------
+    def _tensor(self, a, dtype):
+        import torch
+        if isinstance(a, torch.Tensor):
+            return a.to(device=self.device, dtype=dtype).contiguous()
+        npd = np.complex64 if dtype == torch.complex64 else np.float32
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a), dtype=npd)).to(self.device)
+
+    def _take(self, a, idx, axis, dtype):
+        import torch
+        if isinstance(a, torch.Tensor):
+            it = torch.as_tensor(idx, device=a.device, dtype=torch.long)
+            return a.index_select(axis, it).to(device=self.device, dtype=dtype).contiguous()
+        return self._tensor(np.take(np.asarray(a), idx, axis=axis), dtype)
+
+    @property
+    def stream(self):
+        return _lib.stream_handle(self.device)
+
+    def vol_dims(self):
+        Z, Y, X = self.img_shape
+        return (self.B, self.E, Z, Y, X)
+
+    def _alloc_reduction(self):
+        import torch
+        B, E, Z, Y, X = self.vol_dims()
+        n = int(_lib.load().tr_partial_count(B, Z, Y, X))
+        self._npartial = n
+        self._partial = torch.empty((n, 2), dtype=torch.float64, device=self.device)
+        self._red = torch.empty((2,), dtype=torch.float64, device=self.device)
+
+    def _reduce(self):
+        _lib.call("tr_reduce_partials", self._partial.data_ptr(), self._npartial, self._red.data_ptr(),
+                  self.stream)
+        num, den = self._red.tolist()  # device -> host, synchronises the stream
+        return num, den
+
+    # -- data term -----------------------------------------------------------------
+    def data_dual_update(self, ubar):
+        """q_b <- (q_b + sigma (A ubar_b - y_b)) / (1 + sigma) for every bin (one launch chain per bin)."""
+        s = self.stream
+        sig = float(self.sigma)
+        ops = float(np.float32(1.0 + self.sigma))
+        for b in range(self.B):
+            _lib.call("tr_data_dual_update", self.plan.handle, b, ubar[b].data_ptr(), self.maps.data_ptr(),
+                      self.E, self.C, self.y[b].data_ptr(), self.q[b].data_ptr(), sig, ops, s)
+
+    def data_adjoint(self, g):
+        """g_b = sum_c conj(S_c) NUFFT_b^H(dcf_b q_b,c) for every bin."""
+        s = self.stream
+        for b in range(self.B):
+            _lib.call("tr_data_adjoint", self.plan.handle, b, self.q[b].data_ptr(), self.dcf[b].data_ptr(),
+                      self.maps.data_ptr(), self.E, self.C, g[b].data_ptr(), 0, s)
+
+    def simulate_kspace(self, truth, snr_db=30.0, seed=0, normalize=True):
+        """Replace y by A truth + complex Gaussian noise at snr_db, then y /= max|y|."""
+        import torch
+        x = self._tensor(truth, torch.complex64)
+        s = self.stream
+        ss, cnt = 0.0, 0
+        for b in range(self.B):
+            _lib.call("tr_data_simulate", self.plan.handle, b, x[b].data_ptr(), self.maps.data_ptr(),
+                      self.E, self.C, self.y[b].data_ptr(), s)
+            ss += float(torch.sum(torch.abs(self.y[b]) ** 2))
+            cnt += self.y[b].numel()
+        rms = math.sqrt(ss / max(cnt, 1))
+        sig = rms * 10.0 ** (-snr_db / 20.0) / math.sqrt(2.0)
+        gen = torch.Generator(device=self.device)
+        gen.manual_seed(int(seed))
+        mx = 0.0
+        for b in range(self.B):
+            v = torch.view_as_real(self.y[b])
+            v.add_(torch.randn(v.shape, generator=gen, device=self.device, dtype=torch.float32), alpha=sig)
+            if self.y[b].numel():
+                mx = max(mx, float(torch.max(torch.abs(self.y[b]))))
+        if normalize and mx > 0:
+            for b in range(self.B):
+                self.y[b].div_(np.float32(mx))
+        return mx
+
+    # -- run loop ----------------------------------------------------------------
+    def _converged_value(self, num, den):
+        if math.isfinite(num) and math.isfinite(den) and den > 0.0:
+            return math.sqrt(max(num, 0.0)) / math.sqrt(den)
+        return float("inf")
+
+    def run(self, init=None):
+        """PDHG to the stopping rule; returns ubar (B, E, Z, Y, X) complex64 on the GPU."""
+        import torch
+        with torch.cuda.device(self.device):
+            self.pdhg_init(init)
+            torch.cuda.synchronize(self.device)
+            t0 = time.perf_counter()
+            self.history = []
+            it = -1
+            for it in range(self.max_iter):
+                num, den = self.pdhg(it)
+                tol = self._converged_value(num, den)
+                self.history.append(tol)
+                if tol < self.tol:
+                    logging.info("Converged in %s iterations.", it)
+                    break
+            torch.cuda.synchronize(self.device)
+            self.elapsed = time.perf_counter() - t0
+            self.iterations = it + 1
+            if self.show_pbar:
+                logging.info("[PDHG iterations only] %.2f s (max over ranks)", self.elapsed)
+        return self.ubar

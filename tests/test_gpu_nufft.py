@@ -1,0 +1,135 @@
+"""HIP NUFFT parity against the float64 oracle (rel-L2 <= 1e-5 per operator application)."""
+
+import numpy as np
+import pytest
+
+from oracle import nufft as on
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _coords(rng, shape, M, special=True):
+    half = np.array(shape) / 2.0
+    c = (rng.uniform(-1, 1, (M, 3)) * half).astype(np.float32)
+    if special:
+        c[0] = 0.0                                   # k = 0: integral scaled position, 5 taps
+        c[1] = -half                                 # corner of k-space, wraps
+        c[2] = half * 0.999
+        c[3] = np.round(c[3])                        # on-grid sample
+        c[4] = [0.0, -half[1], half[2]]
+    return c
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _cplx(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.mark.parametrize("shape", [(12, 16, 16), (9, 11, 7), (8, 20, 6), (16, 16, 24)])
+def test_forward_single_image(torch_mod, shape):
+    from toporecon_hip.nufft_backend import create_nufft_backend
+    print("forward single", shape, flush=True)
+    rng = np.random.default_rng(1)
+    coords = [_coords(rng, shape, 1500), _coords(rng, shape, 700)]
+    be = create_nufft_backend("hip", img_shape=shape, coordinates=coords, device=0)
+    img = _cplx(rng, shape)
+    for b in range(2):
+        out = be.forward(img, b).cpu().numpy()
+        ref = on.nufft(img.astype(np.complex128), coords[b])
+        assert out.shape == ref.shape
+        assert _rel(out, ref) < TOL, (b, _rel(out, ref))
+
+
+@pytest.mark.parametrize("shape", [(12, 16, 16), (9, 11, 7), (8, 20, 6)])
+def test_adjoint_single_image(torch_mod, shape):
+    from toporecon_hip.nufft_backend import create_nufft_backend
+    print("adjoint single", shape, flush=True)
+    rng = np.random.default_rng(2)
+    coords = [_coords(rng, shape, 1200).reshape(30, 40, 3)]
+    be = create_nufft_backend("hip", img_shape=shape, coordinates=coords, device=0)
+    k = _cplx(rng, (30, 40))
+    out = be.adjoint(k, 0).cpu().numpy()
+    ref = on.nufft_adjoint(k.astype(np.complex128), coords[0], shape)
+    assert out.shape == tuple(shape)
+    assert _rel(out, ref) < TOL, _rel(out, ref)
+
+
+@pytest.mark.parametrize("E,C", [(2, 3), (6, 32), (1, 8), (3, 12)])
+def test_batched_coils_echoes(torch_mod, E, C):
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    print("batched", E, C, flush=True)
+    shape = (8, 12, 10)
+    rng = np.random.default_rng(E * 100 + C)
+    coords = [_coords(rng, shape, 900)]
+    be = HipNufftBackend(shape, coords, 0)
+    img = _cplx(rng, (E,) + shape)
+    maps = _cplx(rng, (C,) + shape)
+    out = be.forward_batch(img, maps, 0).cpu().numpy()
+    ref = on.nufft(img[:, None].astype(np.complex128) * maps[None], coords[0])
+    assert _rel(out, ref) < TOL
+    k = _cplx(rng, (E, C, 900))
+    dcf = rng.uniform(0.1, 1.0, 900).astype(np.float32)
+    adj = be.adjoint_batch(k, maps, 0, dcf=dcf).cpu().numpy()
+    x = on.nufft_adjoint(k.astype(np.complex128) * dcf, coords[0], shape)
+    ref = np.sum(np.conj(maps)[None] * x, axis=1)
+    assert _rel(adj, ref) < TOL
+
+
+def test_adjointness_and_determinism(torch_mod):
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    shape = (12, 16, 14)
+    rng = np.random.default_rng(7)
+    coords = [_coords(rng, shape, 4000)]
+    be = HipNufftBackend(shape, coords, 0)
+    x = _cplx(rng, shape)
+    y = _cplx(rng, (4000,))
+    Ax = be.forward(x, 0).cpu().numpy().astype(np.complex128)
+    Ahy1 = be.adjoint(y, 0).cpu().numpy()
+    Ahy2 = be.adjoint(y, 0).cpu().numpy()
+    assert np.array_equal(Ahy1.view(np.uint64), Ahy2.view(np.uint64))  # bitwise rerun
+    lhs = np.vdot(Ax, y)
+    rhs = np.vdot(x.astype(np.complex128), Ahy1.astype(np.complex128))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_empty_and_ragged_bins(torch_mod):
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    shape = (8, 8, 8)
+    rng = np.random.default_rng(8)
+    coords = [_coords(rng, shape, 5), np.zeros((0, 3), np.float32), _coords(rng, shape, 333, False)]
+    be = HipNufftBackend(shape, coords, 0)
+    img = _cplx(rng, shape)
+    assert be.forward(img, 1).numel() == 0
+    adj = be.adjoint(np.zeros((0,), np.complex64), 1).cpu().numpy()
+    assert not np.any(adj)
+    for b in (0, 2):
+        ref = on.nufft(img.astype(np.complex128), coords[b])
+        assert _rel(be.forward(img, b).cpu().numpy(), ref) < TOL
+
+
+def test_errors(torch_mod):
+    from toporecon_hip import _lib
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    rng = np.random.default_rng(9)
+    be = HipNufftBackend((8, 8, 8), [_coords(rng, (8, 8, 8), 10)], 0)
+    with pytest.raises(IndexError, match="outside"):
+        be.forward(np.zeros((8, 8, 8), np.complex64), 1)
+    with pytest.raises(ValueError, match="must end in dimension 3"):
+        HipNufftBackend((8, 8, 8), [np.zeros((10, 2), np.float32)], 0)
+    with pytest.raises(ValueError, match="At least one trajectory bin"):
+        HipNufftBackend((8, 8, 8), [], 0)
+    with pytest.raises(_lib.HipLibraryError, match="not supported"):
+        HipNufftBackend((13, 8, 8), [_coords(rng, (13, 8, 8), 10)], 0)  # grid 17 = prime 17

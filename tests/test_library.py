@@ -1,0 +1,45 @@
+"""CPU checks of the native boundary: the library loads and exports every declared symbol."""
+
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "toporecon_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\*?\s+\**(tr_[a-z0-9_]+)\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    from toporecon_hip import _lib
+    assert declared_symbols() == sorted(_lib.exported_symbols())
+
+
+def test_library_loads_and_exports_every_symbol():
+    from toporecon_hip import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libtoporecon_hip.so is not built (python multinodemri-miccai26_amd/build.py)")
+    lib = _lib.load()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert lib.tr_version() == 1
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from toporecon_hip import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(_lib.HipLibraryError):
+        _lib.load()
+
+
+def test_backend_registry_errors():
+    from toporecon_hip.nufft_backend import create_nufft_backend
+    with pytest.raises(ValueError, match="Unknown NUFFT backend"):
+        create_nufft_backend("cufinufft", img_shape=(4, 4, 4), coordinates=[], device=0)
+    with pytest.raises(RuntimeError, match="SigPy is required"):
+        create_nufft_backend(" SigPy ", img_shape=(4, 4, 4), coordinates=[], device=0)
