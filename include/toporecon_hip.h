@@ -37,6 +37,15 @@ typedef struct tr_c64 {
 const char* tr_last_error(void);
 int tr_version(void);
 
+/* Per-kernel timing for measurement (no reference counterpart).  When enabled,
+ * every launch is bracketed by HIP events on its own stream and credited with
+ * its algorithmic bytes; tr_profile_read waits for the recorded events.
+ * tr_profile_enable clears the counters. */
+int tr_profile_enable(int on);
+int tr_profile_num_kernels(void);
+const char* tr_profile_kernel_name(int id);
+int tr_profile_read(int id, double* ms_total, int64_t* launches, double* bytes_total);
+
 /* ---- NUFFT plan (SigPy 0.1.27 nufft defaults: width 4, oversampling 1.25) ----
  * Replaces SigPyNufftBackend.__init__ (src/toporecon/nufft_backend.py:58-88):
  * coords[b] is a device (M_b, 3) float32 array in [z, y, x] SigPy units.  The
@@ -69,6 +78,16 @@ int tr_nufft_forward(tr_plan* plan, int bin, const tr_c64* img, const tr_c64* ma
  * together with the coil loop of tvm.py:216-225; dcf may be NULL, maps == NULL requires C == 1. */
 int tr_nufft_adjoint(tr_plan* plan, int bin, const tr_c64* ksp, const float* dcf, const tr_c64* maps,
                      int E, int C, tr_c64* out, int accumulate, void* stream);
+
+/* FFT-pass access for tests and diagnostics (no reference counterpart):
+ * forward_grid runs the apodise/pad/FFT passes of tr_nufft_forward and copies
+ * the (gz, gy, gx, E*C) grid in FFT-natural order; adjoint_grid runs the
+ * inverse FFT / crop / apodise / coil-sum passes of tr_nufft_adjoint on a
+ * given grid. */
+int tr_nufft_forward_grid(tr_plan* plan, const tr_c64* img, const tr_c64* maps, int E, int C,
+                          tr_c64* grid, void* stream);
+int tr_nufft_adjoint_grid(tr_plan* plan, const tr_c64* grid, const tr_c64* maps, int E, int C,
+                          tr_c64* out, void* stream);
 
 /* Fused data-term dual step of one bin on sorted samples:
  * q <- (q + sigma (A ubar_b - y)) / one_plus_sigma.

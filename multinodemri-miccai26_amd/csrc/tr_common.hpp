@@ -43,6 +43,42 @@ const char* last_error();
 #define TR_LAUNCH_CHECK() TR_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------------------
+// Optional per-kernel timing: when enabled, every launch is bracketed by HIP
+// events on its own stream and credited with its algorithmic bytes (each
+// input read once, each output written once).
+// ---------------------------------------------------------------------------
+enum KernelId : int {
+  kKFwdXPad = 0,
+  kKFftYFwd,
+  kKFftZFwd,
+  kKInterp,
+  kKSpread,
+  kKFftZInv,
+  kKFftYInv,
+  kKAdjXCrop,
+  kKTvmDual,
+  kKTvmPrimal,
+  kKTvmeDual,
+  kKTvmePrimal,
+  kKReduce,
+  kNumKernelIds
+};
+
+const char* kernel_name(int id);
+
+class ProfScope {
+ public:
+  ProfScope(int id, double bytes, hipStream_t s);
+  ~ProfScope();
+
+ private:
+  int id_;
+  double bytes_;
+  hipStream_t s_;
+  void* start_;
+};
+
+// ---------------------------------------------------------------------------
 // Complex helpers (float2 = interleaved complex64).
 // ---------------------------------------------------------------------------
 __host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }

@@ -191,6 +191,94 @@ __device__ __forceinline__ void fft_line(float2* buf, int ld, int ch, int w, con
   }
 }
 
+// ---- compile-time plans ------------------------------------------------------
+// For a length known at compile time every stage's radix, stride and
+// butterflies-per-worker are constants, which keeps the stages at a fraction of
+// the registers of the runtime-radix path above.
+constexpr int kRadixOrder[8] = {8, 4, 2, 3, 5, 7, 11, 13};
+
+constexpr int ct_nstages(int L) {
+  int n = 0;
+  for (int i = 0; i < 8; ++i)
+    while (L % kRadixOrder[i] == 0) {
+      L /= kRadixOrder[i];
+      ++n;
+    }
+  return n;
+}
+
+constexpr int ct_radix(int L, int s) {
+  int k = 0;
+  for (int i = 0; i < 8; ++i)
+    while (L % kRadixOrder[i] == 0) {
+      if (k == s) return kRadixOrder[i];
+      L /= kRadixOrder[i];
+      ++k;
+    }
+  return 1;
+}
+
+constexpr bool ct_supported(int L) {
+  int n = L;
+  for (int i = 0; i < 8; ++i)
+    while (n % kRadixOrder[i] == 0) n /= kRadixOrder[i];
+  return n == 1 && L <= kMaxFftLen;
+}
+
+template <int R, int W, int L, int Ns>
+__device__ __forceinline__ void ct_stage(float2* buf, int ld, int ch, int w, const float2* tw) {
+  constexpr int nb = L / R;
+  constexpr int NB = (nb + W - 1) / W;
+  constexpr int tstride = L / (Ns * R);
+  float2 v[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = w + b * W;
+    if (NB * W == nb || j < nb) {
+      const int jm = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 a = buf[(j + r * nb) * ld + ch];
+        if (r > 0 && Ns > 1) a = cmul(a, tw[jm * r * tstride]);
+        v[b][r] = a;
+      }
+      Dft<R>::run(v[b], tw, L);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = w + b * W;
+    if (NB * W == nb || j < nb) {
+      const int jm = j % Ns;
+      const int base = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * ld + ch] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, int S, int Ns, int W>
+__device__ __forceinline__ void ct_fft(float2* buf, int ld, int ch, int w, const float2* tw) {
+  if constexpr (S < ct_nstages(L)) {
+    constexpr int R = ct_radix(L, S);
+    ct_stage<R, W, L, Ns>(buf, ld, ch, w, tw);
+    ct_fft<L, S + 1, Ns * R, W>(buf, ld, ch, w, tw);
+  }
+}
+
+// L > 0: compile-time plan for length L; L == 0: runtime plan in ax.
+template <int W, int L>
+__device__ __forceinline__ void fft_any(float2* buf, int ld, int ch, int w, const FftAxis& ax,
+                                        const float2* tw) {
+  if constexpr (L > 0) {
+    ct_fft<L, 0, 1, W>(buf, ld, ch, w, tw);
+  } else {
+    fft_line<W>(buf, ld, ch, w, ax, tw);
+  }
+}
+
 // Host: factor L into supported radices (8s first, then 4, 2, odd primes).
 inline bool make_fft_axis(int L, FftAxis* ax) {
   if (L < 1 || L > kMaxFftLen) return false;

@@ -1,0 +1,251 @@
+// Internal declarations shared by the toporecon HIP sources (not part of the C ABI).
+//
+// Grid layout in HBM for one motion bin: [gz][gy][gx][K] complex64 with
+// K = E * C channels (k = e * C + c), in FFT-natural order: image voxel m maps
+// to position (m - n//2) mod g and SigPy grid index t to (t - g//2) mod g, which
+// folds the centred resize / ifftshift / fftshift of sp.nufft into indexing.
+// Samples of a bin are kept sorted by 8^3 grid tile ("sorted order");
+// per-sample data of the fused solver path lives as [M_b][K].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "toporecon_hip.h"
+#include "tr_common.hpp"
+#include "tr_fft.hpp"
+
+namespace tr {
+
+constexpr int kWidth = 4;
+constexpr int kTile = 8;  // sample sort tile edge (grid points)
+constexpr int kMaxKc = 16;  // channels per FFT workgroup
+constexpr int kSpreadSeg = 128;  // CSR entries per heavy-row segment
+
+struct BinData {
+  int64_t M = 0;
+  int64_t nnz = 0;
+  int* perm = nullptr;      // sorted position -> original sample index
+  float4* kp = nullptr;     // sorted scaled coordinates (kz', ky', kx', 0)
+  int* row_ptr = nullptr;   // G + 1, over FFT-natural grid index
+  int* col = nullptr;       // nnz sorted-sample indices
+  float* val = nullptr;     // nnz weights (already divided by width^3)
+  // rows longer than kSpreadSeg, cut into segments of kSpreadSeg entries
+  int64_t nseg = 0;
+  int64_t nheavy = 0;
+  int* seg_row = nullptr;
+  int* seg_j0 = nullptr;
+  int* heavy_row = nullptr;
+  int* heavy_seg_ptr = nullptr;
+};
+
+}  // namespace tr
+
+struct tr_plan {
+  int n[3];   // image z, y, x
+  int g[3];   // oversampled grid
+  int64_t N, G;
+  float beta;
+  float coord_scale[3];
+  float coord_shift[3];
+  float pad_scale;  // 1/sqrt(N)
+  tr::FftAxis ax[3];
+  float2* tw[3] = {nullptr, nullptr, nullptr};
+  float* apod[3] = {nullptr, nullptr, nullptr};
+  std::vector<tr::BinData> bins;
+  float2* work = nullptr;  // grid workspace, G * work_k complex
+  int64_t work_k = 0;
+  float2* partial = nullptr;  // heavy-row segment partial sums
+  int64_t partial_n = 0;
+};
+
+namespace tr {
+
+// ---------------------------------------------------------------------------
+// Kernel arguments
+// ---------------------------------------------------------------------------
+struct GridGeom {
+  int n[3];
+  int g[3];
+};
+
+struct FwdXArgs {
+  GridGeom gg;
+  FftAxis ax;
+  const float2* tw;
+  const float2* img;   // (E, Z, Y, X) for this bin
+  const float2* maps;  // (C, Z, Y, X) or null (C == 1)
+  const float* apod_z;
+  const float* apod_y;
+  const float* apod_x;
+  float scale;
+  int C;
+  int K;  // grid channel count (E * C)
+  float2* grid;
+};
+
+struct LinesArgs {
+  FftAxis ax;
+  const float2* tw;
+  float2* data;
+  int K;
+  int64_t pos_stride;
+  int n_inner;
+  int64_t inner_stride;
+  int64_t outer_stride;
+  int outer_win_n;  // >0: outer index o maps to (o - n//2) mod outer_win_g
+  int outer_win_g;
+  int load_n;   // < L: windowed load (positions (s - n//2) mod L), zeros elsewhere
+  int store_n;  // < L: windowed store
+  int inverse;
+};
+
+struct AdjXArgs {
+  GridGeom gg;
+  FftAxis ax;
+  const float2* tw;
+  const float2* grid;
+  const float2* maps;  // (C, Z, Y, X) or null (C == 1)
+  const float* apod_z;
+  const float* apod_y;
+  const float* apod_x;
+  float scale;
+  int C;
+  int K;
+  float2* out;  // (E, Z, Y, X)
+  int accumulate;
+};
+
+enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2 };
+
+struct InterpArgs {
+  GridGeom gg;
+  float beta;
+  const float4* kp;
+  int64_t M;
+  int K;
+  const float2* grid;
+  int mode;
+  // plain: out[k * M + perm[m]]
+  const int* perm;
+  float2* out;
+  // dual / sim: y[m * K + k], q[m * K + k]
+  const float2* y;
+  float2* q;
+  float sigma;
+  float one_plus_sigma;
+};
+
+struct SpreadArgs {
+  int64_t G;
+  int K;
+  int64_t M;
+  const int* row_ptr;
+  const int* col;
+  const float* val;
+  int sorted_src;  // 1: q (sorted, [M][K]); 0: ksp (original order, [K][M])
+  // sorted source: q[m * K + k] (optionally times dcf_sorted[m])
+  const float2* q;
+  const float* dcf_sorted;
+  // plain source: ksp[k * M + perm[m]] (optionally times dcf_orig[perm[m]])
+  const float2* ksp;
+  const float* dcf_orig;
+  const int* perm;
+  float2* grid;
+  // heavy-row segments
+  int64_t nseg;
+  const int* seg_row;
+  const int* seg_j0;
+  int64_t nheavy;
+  const int* heavy_row;
+  const int* heavy_seg_ptr;
+  float2* partial;  // nseg * K
+};
+
+// Loads of a line tile are staged in registers (all of a thread's loads are
+// issued before the first LDS write) so that every wave keeps up to
+// kMaxFftLen * KC / kBlock loads in flight.
+template <int KC, int LC>
+struct Tile {
+  static constexpr int ld = lds_ld(KC);
+  static constexpr int W = kBlock / KC;  // workers per channel
+  static constexpr int PER = ((LC > 0 ? LC : kMaxFftLen) + W - 1) / W;  // positions per thread
+};
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+inline unsigned grid1d(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+inline int pow2ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Channel chunk of the FFT passes: at most 16 channels per workgroup keeps the
+// Stockham stages under ~130 VGPRs and the LDS tile at L * 17 * 8 bytes.
+inline int fft_kc(int K) { return K >= kMaxKc ? kMaxKc : pow2ceil(K); }
+
+inline size_t fft_smem(int L, int kc) { return sizeof(float2) * (size_t)L * (1 + lds_ld(kc)); }
+
+
+inline GridGeom geom(const tr_plan* p) {
+  GridGeom g;
+  for (int i = 0; i < 3; ++i) {
+    g.n[i] = p->n[i];
+    g.g[i] = p->g[i];
+  }
+  return g;
+}
+
+
+int ensure_partial(tr_plan* p, int64_t n);
+int ensure_work(tr_plan* p, int64_t K);
+int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s);
+void free_plan(tr_plan* p);
+int check_plan_bin(const tr_plan* p, int bin);
+
+// Kernel launchers (one source file per kernel family).
+int launch_fwd_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const FwdXArgs& a);
+int launch_adj_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a);
+int launch_lines(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const LinesArgs& a);
+int launch_interp(const InterpArgs& a, hipStream_t s);
+int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s);
+
+// Oversampled-grid lengths with a compile-time FFT plan for channel chunks of
+// 8 and 16; other (length, chunk) pairs run the runtime-radix kernels.
+#define TR_FAST_LENGTHS(X) X(40) X(50) X(60) X(64) X(80) X(96) X(100) X(120) X(128) X(160) X(200) X(240)
+
+template <class Launch, int KC, class Args>
+inline void launch_len(int L, dim3 g, size_t sm, hipStream_t s, const Args& a) {
+  switch (L) {
+#define TR_CASE_(n)                                 \
+  case n:                                           \
+    Launch::template go<KC, n>(g, sm, s, a);        \
+    return;
+    TR_FAST_LENGTHS(TR_CASE_)
+#undef TR_CASE_
+    default:
+      Launch::template go<KC, 0>(g, sm, s, a);
+  }
+}
+
+template <class Launch, class Args>
+inline int launch_fft_kernel(int kc, int L, dim3 g, size_t sm, hipStream_t s, const Args& a) {
+  switch (kc) {
+    case 1: Launch::template go<1, 0>(g, sm, s, a); break;
+    case 2: Launch::template go<2, 0>(g, sm, s, a); break;
+    case 4: Launch::template go<4, 0>(g, sm, s, a); break;
+    case 8: launch_len<Launch, 8>(L, g, sm, s, a); break;
+    case 16: launch_len<Launch, 16>(L, g, sm, s, a); break;
+    default: TR_FAIL(kUnsupported, "channel chunk must be a power of two <= 16");
+  }
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace tr

@@ -1,0 +1,81 @@
+// FFT passes along y and z of the channel-innermost grid.
+
+#include "tr_internal.hpp"
+
+namespace tr {
+
+// ---------------------------------------------------------------------------
+// Generic pass along y or z: load a (possibly windowed) line of KC channels,
+// FFT (or inverse), store a (possibly windowed) set of positions.
+// Block = (channel chunk, line) with the chunk fastest, so concurrent blocks
+// cover contiguous channel runs of the same positions.
+// ---------------------------------------------------------------------------
+template <int KC, int LC>
+__global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
+  extern __shared__ float2 smem[];
+  using T = Tile<KC, LC>;
+  const int L = LC > 0 ? LC : a.ax.L;
+  float2* tw = smem;
+  float2* buf = smem + L;
+  const int tid = threadIdx.x;
+  const int ch = tid % KC, w = tid / KC;
+  const int nchunk = (a.K + KC - 1) / KC;
+  const int line = blockIdx.x / nchunk;
+  const int chunk = blockIdx.x - line * nchunk;
+  const int o = line / a.n_inner;
+  const int in = line - o * a.n_inner;
+  const int ob = a.outer_win_n > 0 ? pmod(o - a.outer_win_n / 2, a.outer_win_g) : o;
+  const int k0 = chunk * KC;
+  const bool live = k0 + ch < a.K;
+  float2* base = a.data + ob * a.outer_stride + in * a.inner_stride + k0 + ch;
+  for (int i = tid; i < L; i += kBlock) tw[i] = a.tw[i];
+  const bool win_load = a.load_n < L;
+  float2 v[T::PER];
+#pragma unroll
+  for (int it = 0; it < T::PER; ++it) {
+    const int sidx = w + it * T::W;
+    v[it] = make_float2(0.f, 0.f);
+    if (sidx < a.load_n && live) {
+      const int p = win_load ? pmod(sidx - a.load_n / 2, L) : sidx;
+      v[it] = base[p * a.pos_stride];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < T::PER; ++it) {
+    const int sidx = w + it * T::W;
+    if (sidx < L) {
+      const int p = sidx < a.load_n ? (win_load ? pmod(sidx - a.load_n / 2, L) : sidx)
+                                    : (a.load_n - a.load_n / 2) + (sidx - a.load_n);
+      float2 x = v[it];
+      if (a.inverse) x.y = -x.y;
+      buf[p * T::ld + ch] = x;
+    }
+  }
+  __syncthreads();
+  fft_any<T::W, LC>(buf, T::ld, ch, w, a.ax, tw);
+  if (!live) return;
+  const bool win_store = a.store_n < L;
+#pragma unroll
+  for (int it = 0; it < T::PER; ++it) {
+    const int sidx = w + it * T::W;
+    if (sidx < a.store_n) {
+      const int p = win_store ? pmod(sidx - a.store_n / 2, L) : sidx;
+      float2 x = buf[p * T::ld + ch];
+      if (a.inverse) x.y = -x.y;
+      base[p * a.pos_stride] = x;
+    }
+  }
+}
+
+struct LinesLaunch {
+  template <int KC, int LC>
+  static void go(dim3 g, size_t sm, hipStream_t s, const LinesArgs& a) {
+    hipLaunchKernelGGL((k_fft_lines<KC, LC>), g, dim3(kBlock), sm, s, a);
+  }
+};
+
+int launch_lines(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const LinesArgs& a) {
+  return launch_fft_kernel<LinesLaunch>(kc, L, grid, smem, s, a);
+}
+
+}  // namespace tr

@@ -274,6 +274,7 @@ int tr_tvm_dual(int B, int E, int Z, int Y, int X, const tr_c64* ubar, tr_c64* p
                 float lam, int l2_coupling, void* stream) {
   VolGeom g;
   if (int r = check_geom(B, E, Z, Y, X, &g)) return r;
+  ProfScope ps(kKTvmDual, 3.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
   hipLaunchKernelGGL(k_tvm_dual, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(ubar),
                      F2(p_m), sigma, lam, l2_coupling);
   TR_LAUNCH_CHECK();
@@ -284,6 +285,7 @@ int tr_tvm_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, co
                   const tr_c64* p_m, float tau, double* partial, void* stream) {
   VolGeom g;
   if (int r = check_geom(B, E, Z, Y, X, &g)) return r;
+  ProfScope ps(kKTvmPrimal, 5.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
   hipLaunchKernelGGL(k_tvm_primal, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(u),
                      F2(ubar), F2(grad), F2(p_m), tau, (double2*)partial);
   TR_LAUNCH_CHECK();
@@ -295,6 +297,7 @@ int tr_tvme_dual(int B, int E, int Z, int Y, int X, const tr_c64* ubar, tr_c64* 
                  void* stream) {
   VolGeom g;
   if (int r = check_geom(B, E, Z, Y, X, &g)) return r;
+  ProfScope ps(kKTvmeDual, 9.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
   hipLaunchKernelGGL(k_tvme_dual, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(ubar),
                      F2(p_m), F2(p_ex), F2(p_ey), F2(p_ez), sigma, lam_m, lam_e, l2_coupling);
   TR_LAUNCH_CHECK();
@@ -306,6 +309,7 @@ int tr_tvme_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, c
                    float tau, double* partial, void* stream) {
   VolGeom g;
   if (int r = check_geom(B, E, Z, Y, X, &g)) return r;
+  ProfScope ps(kKTvmePrimal, 8.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
   hipLaunchKernelGGL(k_tvme_primal, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(u),
                      F2(ubar), F2(grad), F2(p_m), F2(p_ex), F2(p_ey), F2(p_ez), tau, (double2*)partial);
   TR_LAUNCH_CHECK();
@@ -314,6 +318,7 @@ int tr_tvme_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, c
 
 int tr_reduce_partials(const double* partial, int64_t n, double* out2, void* stream) {
   if (n < 1 || n > INT32_MAX) TR_FAIL(kInvalidArgument, "bad partial count");
+  ProfScope ps(kKReduce, 16.0 * n + 16.0, (hipStream_t)stream);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kEwBlock), 0, (hipStream_t)stream,
                      (const double2*)partial, (int)n, out2);
   TR_LAUNCH_CHECK();

@@ -1,0 +1,437 @@
+// NUFFT plan: coordinate scaling (SigPy float32 semantics), tile sort of the
+// samples, the deterministic spreading CSR, tables and workspace management.
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+#include "tr_internal.hpp"
+
+namespace tr {
+
+// ---------------------------------------------------------------------------
+// Plan construction kernels
+// ---------------------------------------------------------------------------
+__global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh, int3 g, int3 nt,
+                             float4* kp, unsigned* key, int* idx) {
+  // k' = fl32(fl32(k * s) + shift), as SigPy's _scale_coord on a float32 trajectory:
+  // the product must be rounded before the add (no FMA), or a coordinate that
+  // SigPy puts exactly on a grid line (5 taps) lands 1 ulp off it (4 taps).
+#pragma clang fp contract(off)
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const float pz = coord[3 * m + 0] * s.x;
+  const float py = coord[3 * m + 1] * s.y;
+  const float px = coord[3 * m + 2] * s.z;
+  const float kz = pz + sh.x;
+  const float ky = py + sh.y;
+  const float kx = px + sh.z;
+  kp[m] = make_float4(kz, ky, kx, 0.f);
+  auto tile = [](float k, int g, int n) {
+    int t = (int)floorf((k + 2.0f) / (float)kTile);
+    return t < 0 ? 0 : (t >= n ? n - 1 : t);
+  };
+  const int tz = tile(kz, g.x, nt.x), ty = tile(ky, g.y, nt.y), tx = tile(kx, g.z, nt.z);
+  key[m] = (unsigned)((tz * nt.y + ty) * nt.z + tx);
+  idx[m] = (int)m;
+}
+
+__global__ void k_gather_kp(const float4* kp_in, const int* perm, int64_t M, float4* kp_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M) kp_out[j] = kp_in[perm[j]];
+}
+
+__device__ __forceinline__ int tap_count(float k) {
+  if (!isfinite(k) || fabsf(k) > 1.0e6f) return 0;  // same rule as axis_taps
+  const double kd = (double)k;
+  return (int)floor(kd + 2.0) - (int)ceil(kd - 2.0) + 1;
+}
+
+__global__ void k_count_taps(const float4* kp, int64_t M, int* cnt) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const float4 k = kp[j];
+  cnt[j] = tap_count(k.x) * tap_count(k.y) * tap_count(k.z);
+}
+
+__global__ void k_fill_pairs(const float4* kp, int64_t M, const int* off, int3 g, float beta,
+                             unsigned* key, unsigned long long* val) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const float4 k = kp[j];
+  float wz[5], wy[5], wx[5];
+  int pz[5], py[5], px[5];
+  const int nz = axis_taps(k.x, g.x, beta, 1.0f, wz, pz);
+  const int ny = axis_taps(k.y, g.y, beta, 1.0f, wy, py);
+  const int nx = axis_taps(k.z, g.z, beta, 1.0f / (kWidth * kWidth * kWidth), wx, px);
+  int p = off[j];
+  for (int dz = 0; dz < nz; ++dz)
+    for (int dy = 0; dy < ny; ++dy) {
+      const float wzy = wz[dz] * wy[dy];
+      for (int dx = 0; dx < nx; ++dx) {
+        const float w = wzy * wx[dx];
+        key[p] = (unsigned)(((int64_t)pz[dz] * g.y + py[dy]) * g.z + px[dx]);
+        val[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned long long)__float_as_uint(w);
+        ++p;
+      }
+    }
+}
+
+__global__ void k_unpack_pairs(const unsigned long long* val, int64_t nnz, int* col, float* w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const unsigned long long v = val[i];
+  col[i] = (int)(v >> 32);
+  w[i] = __uint_as_float((unsigned)(v & 0xffffffffull));
+}
+
+__global__ void k_row_ptr(const unsigned* key, int64_t nnz, int64_t G, int* row_ptr) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > G) return;
+  // lower_bound(key, t)
+  int64_t lo = 0, hi = nnz;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)key[mid] < t) lo = mid + 1; else hi = mid;
+  }
+  row_ptr[t] = (int)lo;
+}
+
+template <typename T>
+__global__ void k_gather_samples(const T* src, const int* perm, int64_t M, int K, T* dst) {
+  // dst[j * K + k] = src[k * M + perm[j]]
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * K) return;
+  const int64_t j = i / K;
+  const int k = (int)(i - j * K);
+  dst[i] = src[(int64_t)k * M + perm[j]];
+}
+
+template <typename T>
+__global__ void k_scatter_samples(const T* src, const int* perm, int64_t M, int K, T* dst) {
+  // dst[k * M + perm[j]] = src[j * K + k]
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * K) return;
+  const int64_t j = i / K;
+  const int k = (int)(i - j * K);
+  dst[(int64_t)k * M + perm[j]] = src[i];
+}
+
+int ensure_partial(tr_plan* p, int64_t n) {
+  if (p->partial_n >= n) return kOk;
+  if (p->partial) {
+    TR_HIP(hipDeviceSynchronize());
+    TR_HIP(hipFree(p->partial));
+    p->partial = nullptr;
+    p->partial_n = 0;
+  }
+  TR_HIP(hipMalloc(&p->partial, sizeof(float2) * n));
+  p->partial_n = n;
+  return kOk;
+}
+
+int ensure_work(tr_plan* p, int64_t K) {
+  if (p->work_k >= K) return kOk;
+  if (p->work) {
+    TR_HIP(hipDeviceSynchronize());
+    TR_HIP(hipFree(p->work));
+    p->work = nullptr;
+    p->work_k = 0;
+  }
+  TR_HIP(hipMalloc(&p->work, sizeof(float2) * p->G * K));
+  p->work_k = K;
+  return kOk;
+}
+
+int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s) {
+  b.M = M;
+  TR_HIP(hipMalloc(&b.row_ptr, sizeof(int) * (p->G + 1)));
+  if (M == 0) {
+    TR_HIP(hipMemsetAsync(b.row_ptr, 0, sizeof(int) * (p->G + 1), s));
+    return kOk;
+  }
+  // 125 taps per sample must index an int32 CSR.
+  if (M > (int64_t)17000000) TR_FAIL(kUnsupported, "more than 17M samples in one bin");
+  float4* kp_raw = nullptr;
+  unsigned *key_in = nullptr, *key_out = nullptr;
+  int* idx = nullptr;
+  TR_HIP(hipMalloc(&kp_raw, sizeof(float4) * M));
+  TR_HIP(hipMalloc(&key_in, sizeof(unsigned) * M));
+  TR_HIP(hipMalloc(&key_out, sizeof(unsigned) * M));
+  TR_HIP(hipMalloc(&idx, sizeof(int) * M));
+  TR_HIP(hipMalloc(&b.perm, sizeof(int) * M));
+  TR_HIP(hipMalloc(&b.kp, sizeof(float4) * M));
+  int3 nt;
+  nt.x = p->g[0] / kTile + 2;
+  nt.y = p->g[1] / kTile + 2;
+  nt.z = p->g[2] / kTile + 2;
+  const int3 g3 = make_int3(p->g[0], p->g[1], p->g[2]);
+  hipLaunchKernelGGL(k_scale_keys, dim3(grid1d(M, 256)), dim3(256), 0, s, coord, M,
+                     make_float3(p->coord_scale[0], p->coord_scale[1], p->coord_scale[2]),
+                     make_float3(p->coord_shift[0], p->coord_shift[1], p->coord_shift[2]), g3, nt,
+                     kp_raw, key_in, idx);
+  TR_LAUNCH_CHECK();
+  int key_bits = 1;
+  while (((int64_t)1 << key_bits) < (int64_t)nt.x * nt.y * nt.z) ++key_bits;
+  size_t tmp_bytes = 0;
+  TR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key_in, key_out, idx, b.perm, (int)M, 0,
+                                            key_bits, s));
+  void* tmp = nullptr;
+  TR_HIP(hipMalloc(&tmp, tmp_bytes));
+  TR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key_in, key_out, idx, b.perm, (int)M, 0,
+                                            key_bits, s));
+  hipLaunchKernelGGL(k_gather_kp, dim3(grid1d(M, 256)), dim3(256), 0, s, kp_raw, b.perm, M, b.kp);
+  TR_LAUNCH_CHECK();
+  // tap counts -> offsets
+  int* cnt = nullptr;
+  int* off = nullptr;
+  TR_HIP(hipMalloc(&cnt, sizeof(int) * (M + 1)));
+  TR_HIP(hipMalloc(&off, sizeof(int) * (M + 1)));
+  TR_HIP(hipMemsetAsync(cnt + M, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_count_taps, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, cnt);
+  TR_LAUNCH_CHECK();
+  size_t scan_bytes = 0;
+  TR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, (int)(M + 1), s));
+  void* scan_tmp = nullptr;
+  TR_HIP(hipMalloc(&scan_tmp, scan_bytes));
+  TR_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt, off, (int)(M + 1), s));
+  int nnz_i = 0;
+  TR_HIP(hipMemcpyAsync(&nnz_i, off + M, sizeof(int), hipMemcpyDeviceToHost, s));
+  TR_HIP(hipStreamSynchronize(s));
+  const int64_t nnz = nnz_i;
+  b.nnz = nnz;
+  if (nnz == 0) {
+    TR_HIP(hipMemsetAsync(b.row_ptr, 0, sizeof(int) * (p->G + 1), s));
+    TR_HIP(hipStreamSynchronize(s));
+    void* frees0[] = {kp_raw, key_in, key_out, idx, tmp, cnt, off, scan_tmp};
+    for (void* f : frees0) TR_HIP(hipFree(f));
+    return kOk;
+  }
+  unsigned *pk_in = nullptr, *pk_out = nullptr;
+  unsigned long long *pv_in = nullptr, *pv_out = nullptr;
+  TR_HIP(hipMalloc(&pk_in, sizeof(unsigned) * nnz));
+  TR_HIP(hipMalloc(&pk_out, sizeof(unsigned) * nnz));
+  TR_HIP(hipMalloc(&pv_in, sizeof(unsigned long long) * nnz));
+  TR_HIP(hipMalloc(&pv_out, sizeof(unsigned long long) * nnz));
+  hipLaunchKernelGGL(k_fill_pairs, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, off, g3, p->beta,
+                     pk_in, pv_in);
+  TR_LAUNCH_CHECK();
+  int gbits = 1;
+  while (((int64_t)1 << gbits) < p->G) ++gbits;
+  size_t tmp2_bytes = 0;
+  TR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2_bytes, pk_in, pk_out, pv_in, pv_out,
+                                            (int)nnz, 0, gbits, s));
+  void* tmp2 = nullptr;
+  TR_HIP(hipMalloc(&tmp2, tmp2_bytes));
+  TR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp2, tmp2_bytes, pk_in, pk_out, pv_in, pv_out, (int)nnz,
+                                            0, gbits, s));
+  TR_HIP(hipMalloc(&b.col, sizeof(int) * nnz));
+  TR_HIP(hipMalloc(&b.val, sizeof(float) * nnz));
+  hipLaunchKernelGGL(k_unpack_pairs, dim3(grid1d(nnz, 256)), dim3(256), 0, s, pv_out, nnz, b.col,
+                     b.val);
+  TR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_row_ptr, dim3(grid1d(p->G + 1, 256)), dim3(256), 0, s, pk_out, nnz, p->G,
+                     b.row_ptr);
+  TR_LAUNCH_CHECK();
+  TR_HIP(hipStreamSynchronize(s));
+  {  // heavy rows -> fixed segments of kSpreadSeg entries
+    std::vector<int> rp(p->G + 1);
+    TR_HIP(hipMemcpy(rp.data(), b.row_ptr, sizeof(int) * (p->G + 1), hipMemcpyDeviceToHost));
+    std::vector<int> seg_row, seg_j0, heavy_row, heavy_ptr;
+    for (int64_t t = 0; t < p->G; ++t) {
+      const int len = rp[t + 1] - rp[t];
+      if (len <= kSpreadSeg) continue;
+      heavy_row.push_back((int)t);
+      heavy_ptr.push_back((int)seg_row.size());
+      for (int j = rp[t]; j < rp[t + 1]; j += kSpreadSeg) {
+        seg_row.push_back((int)t);
+        seg_j0.push_back(j);
+      }
+    }
+    heavy_ptr.push_back((int)seg_row.size());
+    b.nseg = (int64_t)seg_row.size();
+    b.nheavy = (int64_t)heavy_row.size();
+    if (b.nseg > 0) {
+      TR_HIP(hipMalloc(&b.seg_row, sizeof(int) * b.nseg));
+      TR_HIP(hipMalloc(&b.seg_j0, sizeof(int) * b.nseg));
+      TR_HIP(hipMalloc(&b.heavy_row, sizeof(int) * b.nheavy));
+      TR_HIP(hipMalloc(&b.heavy_seg_ptr, sizeof(int) * (b.nheavy + 1)));
+      TR_HIP(hipMemcpy(b.seg_row, seg_row.data(), sizeof(int) * b.nseg, hipMemcpyHostToDevice));
+      TR_HIP(hipMemcpy(b.seg_j0, seg_j0.data(), sizeof(int) * b.nseg, hipMemcpyHostToDevice));
+      TR_HIP(hipMemcpy(b.heavy_row, heavy_row.data(), sizeof(int) * b.nheavy, hipMemcpyHostToDevice));
+      TR_HIP(hipMemcpy(b.heavy_seg_ptr, heavy_ptr.data(), sizeof(int) * (b.nheavy + 1),
+                       hipMemcpyHostToDevice));
+    }
+  }
+  void* frees[] = {kp_raw, key_in, key_out, idx, tmp, cnt, off, scan_tmp, pk_in, pk_out, pv_in, pv_out, tmp2};
+  for (void* f : frees) TR_HIP(hipFree(f));
+  return kOk;
+}
+
+void free_plan(tr_plan* p) {
+  if (!p) return;
+  for (auto& b : p->bins) {
+    (void)hipFree(b.perm);
+    (void)hipFree(b.kp);
+    (void)hipFree(b.row_ptr);
+    (void)hipFree(b.col);
+    (void)hipFree(b.val);
+    (void)hipFree(b.seg_row);
+    (void)hipFree(b.seg_j0);
+    (void)hipFree(b.heavy_row);
+    (void)hipFree(b.heavy_seg_ptr);
+  }
+  for (int i = 0; i < 3; ++i) {
+    (void)hipFree(p->tw[i]);
+    (void)hipFree(p->apod[i]);
+  }
+  (void)hipFree(p->work);
+  (void)hipFree(p->partial);
+  delete p;
+}
+
+int check_plan_bin(const tr_plan* p, int bin) {
+  if (!p) TR_FAIL(kInvalidArgument, "null plan");
+  if (bin < 0 || bin >= (int)p->bins.size())
+    TR_FAIL(kInvalidArgument, "bin_index " + std::to_string(bin) + " is outside [0, " +
+                                  std::to_string(p->bins.size()) + ").");
+  return kOk;
+}
+
+}  // namespace tr
+
+using namespace tr;
+
+extern "C" {
+
+int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
+                   const int64_t* samples_per_bin, void* stream) {
+  if (!out) TR_FAIL(kInvalidArgument, "null output pointer");
+  *out = nullptr;
+  if (nz < 1 || ny < 1 || nx < 1) TR_FAIL(kInvalidArgument, "img_shape must be three positive values");
+  if (nbins < 1) TR_FAIL(kInvalidArgument, "At least one trajectory bin is required.");
+  hipStream_t s = (hipStream_t)stream;
+  tr_plan* p = new tr_plan();
+  const int n[3] = {nz, ny, nx};
+  const double beta = M_PI * std::sqrt(std::pow((kWidth / 1.25) * (1.25 - 0.5), 2) - 0.8);
+  p->beta = (float)beta;
+  p->N = (int64_t)nz * ny * nx;
+  p->pad_scale = (float)(1.0 / std::sqrt((double)p->N));
+  for (int i = 0; i < 3; ++i) {
+    p->n[i] = n[i];
+    p->g[i] = (int)std::ceil(1.25 * n[i]);
+    p->coord_scale[i] = (float)((double)p->g[i] / n[i]);
+    p->coord_shift[i] = (float)(p->g[i] / 2);
+    if (!make_fft_axis(p->g[i], &p->ax[i])) {
+      free_plan(p);
+      TR_FAIL(kUnsupported, "oversampled grid length " + std::to_string(p->g[i]) +
+                                " is not supported (max 256, radices 2,3,5,7,11,13)");
+    }
+  }
+  p->G = (int64_t)p->g[0] * p->g[1] * p->g[2];
+  for (int i = 0; i < 3; ++i) {
+    const int L = p->g[i];
+    std::vector<float2> tw(L);
+    for (int t = 0; t < L; ++t) {
+      const double a = -2.0 * M_PI * (double)t / L;
+      tw[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    std::vector<float> ap(n[i]);
+    for (int j = 0; j < n[i]; ++j) {
+      const double x = M_PI * kWidth * (double)(j - n[i] / 2) / p->g[i];
+      const double sarg = std::sqrt(beta * beta - x * x);
+      ap[j] = (float)(sarg / std::sinh(sarg));
+    }
+    if (hipMalloc(&p->tw[i], sizeof(float2) * L) != hipSuccess ||
+        hipMalloc(&p->apod[i], sizeof(float) * n[i]) != hipSuccess) {
+      free_plan(p);
+      TR_FAIL(kOutOfMemory, "allocating plan tables");
+    }
+    (void)hipMemcpy(p->tw[i], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice);
+    (void)hipMemcpy(p->apod[i], ap.data(), sizeof(float) * n[i], hipMemcpyHostToDevice);
+  }
+  p->bins.resize(nbins);
+  for (int b = 0; b < nbins; ++b) {
+    if (samples_per_bin[b] < 0) {
+      free_plan(p);
+      TR_FAIL(kInvalidArgument, "negative sample count");
+    }
+    const int r = build_bin(p, p->bins[b], coords[b], samples_per_bin[b], s);
+    if (r) {
+      std::string msg = last_error();
+      free_plan(p);
+      set_error(msg);
+      return r;
+    }
+  }
+  *out = p;
+  return kOk;
+}
+
+int tr_plan_destroy(tr_plan* plan) {
+  if (plan) {
+    (void)hipDeviceSynchronize();
+    free_plan(plan);
+  }
+  return kOk;
+}
+
+int tr_plan_info(const tr_plan* p, int bin, int64_t* info) {
+  if (!p || !info) TR_FAIL(kInvalidArgument, "null argument");
+  for (int i = 0; i < 3; ++i) {
+    info[i] = p->n[i];
+    info[3 + i] = p->g[i];
+  }
+  info[6] = (int64_t)p->bins.size();
+  if (bin >= 0 && bin < (int)p->bins.size()) {
+    info[7] = p->bins[bin].M;
+    info[8] = p->bins[bin].nnz;
+  } else {
+    info[7] = info[8] = -1;
+  }
+  info[9] = p->work_k;
+  return kOk;
+}
+
+int tr_plan_release_workspace(tr_plan* p) {
+  if (!p) TR_FAIL(kInvalidArgument, "null plan");
+  if (p->work) {
+    TR_HIP(hipDeviceSynchronize());
+    TR_HIP(hipFree(p->work));
+  }
+  p->work = nullptr;
+  p->work_k = 0;
+  return kOk;
+}
+
+int tr_plan_gather_samples(const tr_plan* p, int bin, const tr_c64* src, int K, tr_c64* dst, void* stream) {
+  if (int r = check_plan_bin(p, bin)) return r;
+  const BinData& b = p->bins[bin];
+  if (b.M == 0 || K == 0) return kOk;
+  hipLaunchKernelGGL(k_gather_samples<float2>, dim3(grid1d(b.M * K, 256)), dim3(256), 0,
+                     (hipStream_t)stream, F2(src), b.perm, b.M, K, F2(dst));
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_plan_scatter_samples(const tr_plan* p, int bin, const tr_c64* src, int K, tr_c64* dst, void* stream) {
+  if (int r = check_plan_bin(p, bin)) return r;
+  const BinData& b = p->bins[bin];
+  if (b.M == 0 || K == 0) return kOk;
+  hipLaunchKernelGGL(k_scatter_samples<float2>, dim3(grid1d(b.M * K, 256)), dim3(256), 0,
+                     (hipStream_t)stream, F2(src), b.perm, b.M, K, F2(dst));
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_plan_gather_weights(const tr_plan* p, int bin, const float* src, float* dst, void* stream) {
+  if (int r = check_plan_bin(p, bin)) return r;
+  const BinData& b = p->bins[bin];
+  if (b.M == 0) return kOk;
+  hipLaunchKernelGGL(k_gather_samples<float>, dim3(grid1d(b.M, 256)), dim3(256), 0,
+                     (hipStream_t)stream, src, b.perm, b.M, 1, dst);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // extern "C"

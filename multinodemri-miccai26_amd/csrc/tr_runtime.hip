@@ -1,0 +1,113 @@
+// Error reporting, per-kernel timing and version entry points.
+
+#include <mutex>
+
+#include "tr_internal.hpp"
+
+namespace tr {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+// ---- kernel timing -----------------------------------------------------------
+namespace {
+struct ProfPending {
+  hipEvent_t a, b;
+  int id;
+  double bytes;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<hipEvent_t> g_prof_pool;
+std::vector<ProfPending> g_prof_pending;
+double g_prof_ms[kNumKernelIds];
+double g_prof_bytes[kNumKernelIds];
+int64_t g_prof_count[kNumKernelIds];
+
+hipEvent_t pool_get() {
+  if (!g_prof_pool.empty()) {
+    hipEvent_t e = g_prof_pool.back();
+    g_prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void prof_drain() {
+  for (auto& p : g_prof_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      g_prof_ms[p.id] += ms;
+      g_prof_bytes[p.id] += p.bytes;
+      g_prof_count[p.id] += 1;
+    }
+    g_prof_pool.push_back(p.a);
+    g_prof_pool.push_back(p.b);
+  }
+  g_prof_pending.clear();
+}
+}  // namespace
+
+const char* kernel_name(int id) {
+  static const char* names[kNumKernelIds] = {
+      "fwd_x_pad", "fft_y_fwd", "fft_z_fwd", "interp", "spread", "fft_z_inv", "fft_y_inv",
+      "adj_x_crop", "tvm_dual", "tvm_primal", "tvme_dual", "tvme_primal", "reduce"};
+  return (id >= 0 && id < kNumKernelIds) ? names[id] : "";
+}
+
+ProfScope::ProfScope(int id, double bytes, hipStream_t s) : id_(id), bytes_(bytes), s_(s), start_(nullptr) {
+  if (!g_prof_on) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  hipEvent_t e = pool_get();
+  if (e && hipEventRecord(e, s) == hipSuccess) start_ = e;
+}
+
+ProfScope::~ProfScope() {
+  if (!start_) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  hipEvent_t e = pool_get();
+  if (e && hipEventRecord(e, s_) == hipSuccess) {
+    g_prof_pending.push_back({(hipEvent_t)start_, e, id_, bytes_});
+  }
+}
+
+}  // namespace tr
+
+using namespace tr;
+
+extern "C" {
+
+const char* tr_last_error(void) { return tr::last_error(); }
+
+int tr_version(void) { return TR_ABI_VERSION; }
+
+int tr_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  prof_drain();
+  for (int i = 0; i < kNumKernelIds; ++i) {
+    g_prof_ms[i] = 0.0;
+    g_prof_bytes[i] = 0.0;
+    g_prof_count[i] = 0;
+  }
+  g_prof_on = on != 0;
+  return kOk;
+}
+
+int tr_profile_num_kernels(void) { return kNumKernelIds; }
+
+const char* tr_profile_kernel_name(int id) { return kernel_name(id); }
+
+int tr_profile_read(int id, double* ms_total, int64_t* launches, double* bytes_total) {
+  if (id < 0 || id >= kNumKernelIds) TR_FAIL(kInvalidArgument, "bad kernel id");
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  prof_drain();
+  if (ms_total) *ms_total = g_prof_ms[id];
+  if (launches) *launches = g_prof_count[id];
+  if (bytes_total) *bytes_total = g_prof_bytes[id];
+  return kOk;
+}
+
+}  // extern "C"

@@ -24,6 +24,10 @@ c_dp = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "tr_last_error": (ctypes.c_char_p, []),
     "tr_version": (c_int, []),
+    "tr_profile_enable": (c_int, [c_int]),
+    "tr_profile_num_kernels": (c_int, []),
+    "tr_profile_kernel_name": (ctypes.c_char_p, [c_int]),
+    "tr_profile_read": (c_int, [c_int, c_dp, ctypes.POINTER(c_i64), c_dp]),
     "tr_plan_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int,
                                ctypes.POINTER(c_vp), ctypes.POINTER(c_i64), c_vp]),
     "tr_plan_destroy": (c_int, [c_vp]),
@@ -34,6 +38,8 @@ SIGNATURES = {
     "tr_plan_gather_weights": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
     "tr_nufft_forward": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
     "tr_nufft_adjoint": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_int, c_vp]),
+    "tr_nufft_forward_grid": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
+    "tr_nufft_adjoint_grid": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
     "tr_data_dual_update": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_f32, c_f32,
                                     c_vp]),
     "tr_data_simulate": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
@@ -71,6 +77,24 @@ def load():
             fn.argtypes = args
         _lib = lib
         return lib
+
+
+def profile_enable(on: bool = True) -> None:
+    call("tr_profile_enable", int(bool(on)))
+
+
+def profile_read() -> dict:
+    """{kernel name: (total ms, launches, algorithmic bytes)} since the last enable."""
+    lib = load()
+    out = {}
+    for i in range(lib.tr_profile_num_kernels()):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        by = ctypes.c_double()
+        check(lib.tr_profile_read(i, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by)), "tr_profile_read")
+        if n.value:
+            out[lib.tr_profile_kernel_name(i).decode()] = (ms.value, n.value, by.value)
+    return out
 
 
 def exported_symbols():

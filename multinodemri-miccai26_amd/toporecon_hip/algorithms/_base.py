@@ -40,7 +40,8 @@ class HipPdhgBase:
     algorithm = "base"
 
     def _setup(self, ksp, coord, dcf, mps, resp, dual_q, B_local, *, margin, device, B_total, b0,
-               b1, bin_edges, comms, show_pbar, nufft_backend, sigma, tau, max_iter, tol):
+               b1, bin_edges, comms, show_pbar, nufft_backend, sigma, tau, max_iter, tol,
+               echoes=None):
         import torch
         if str(nufft_backend).strip().lower() != "hip":
             raise ValueError(f"the HIP reconstructors run the 'hip' NUFFT backend, got {nufft_backend!r}")
@@ -61,7 +62,9 @@ class HipPdhgBase:
         self.tol = tol
         self.show_pbar = show_pbar
         self.C = int(mps.shape[0])
-        self.E = int(ksp.shape[0])
+        if ksp is None and echoes is None:
+            raise ValueError("ksp is required (or echoes= for a solver whose k-space is simulated)")
+        self.E = int(ksp.shape[0]) if ksp is not None else int(echoes)
         self.K = self.E * self.C
         self.img_shape = [int(v) for v in mps.shape[-3:]]
 
@@ -88,11 +91,14 @@ class HipPdhgBase:
         stream = _lib.stream_handle(dev)
         for b, idx in enumerate(members):
             M = self.plan.samples(b)
-            k_b = self._take(ksp, idx, axis=2, dtype=torch.complex64).reshape(self.K, M)
-            y_s = torch.empty((M, self.K), dtype=torch.complex64, device=dev)
-            _lib.call("tr_plan_gather_samples", self.plan.handle, b, k_b.data_ptr(), self.K,
-                      y_s.data_ptr(), stream)
-            del k_b
+            if ksp is None:
+                y_s = torch.zeros((M, self.K), dtype=torch.complex64, device=dev)
+            else:
+                k_b = self._take(ksp, idx, axis=2, dtype=torch.complex64).reshape(self.K, M)
+                y_s = torch.empty((M, self.K), dtype=torch.complex64, device=dev)
+                _lib.call("tr_plan_gather_samples", self.plan.handle, b, k_b.data_ptr(), self.K,
+                          y_s.data_ptr(), stream)
+                del k_b
             q_s = torch.zeros((M, self.K), dtype=torch.complex64, device=dev)
             if dual_q is not None:
                 q_b = self._take(dual_q, idx, axis=2, dtype=torch.complex64).reshape(self.K, M)
@@ -196,8 +202,13 @@ class HipPdhgBase:
             return math.sqrt(max(num, 0.0)) / math.sqrt(den)
         return float("inf")
 
-    def run(self, init=None):
-        """PDHG to the stopping rule; returns ubar (B, E, Z, Y, X) complex64 on the GPU."""
+    def run(self, init=None, progress_every=0):
+        """PDHG to the stopping rule; returns ubar (B, E, Z, Y, X) complex64 on the GPU.
+
+        progress_every > 0 prints the iteration and stopping value to stderr
+        every that many iterations.
+        """
+        import sys
         import torch
         with torch.cuda.device(self.device):
             self.pdhg_init(init)
@@ -209,6 +220,9 @@ class HipPdhgBase:
                 num, den = self.pdhg(it)
                 tol = self._converged_value(num, den)
                 self.history.append(tol)
+                if progress_every and (it + 1) % progress_every == 0:
+                    print(f"[pdhg] it={it + 1} tol={tol:.3e} t={time.perf_counter() - t0:.1f}s",
+                          file=sys.stderr, flush=True)
                 if tol < self.tol:
                     logging.info("Converged in %s iterations.", it)
                     break

@@ -24,14 +24,15 @@ class TvmReconstructor(HipPdhgBase):
                  sigma=0.01, tau=0.01, max_iter=300, tol=1e-3, margin=10, device=0, B_total=None,
                  b0=None, b1=None, motion_left_peer=None, motion_right_peer=None, bin_edges=None,
                  world_comm=None, group_comm=None, leader_comm=None, nccl_group=None, dist_grid=None,
-                 tagger=None, show_pbar=True, reduce_per_bin=False, nufft_backend="hip"):
+                 tagger=None, show_pbar=True, reduce_per_bin=False, nufft_backend="hip", echoes=None):
         self.lamda = lamda
         self.l2_coupling = bool(l2_coupling)
         self.reduce_per_bin = reduce_per_bin
         self._setup(ksp, coord, dcf, mps, resp, dual_q, B_local, margin=margin, device=device,
                     B_total=B_total, b0=b0, b1=b1, bin_edges=bin_edges,
                     comms=(world_comm, group_comm, leader_comm), show_pbar=show_pbar,
-                    nufft_backend=nufft_backend, sigma=sigma, tau=tau, max_iter=max_iter, tol=tol)
+                    nufft_backend=nufft_backend, sigma=sigma, tau=tau, max_iter=max_iter, tol=tol,
+                    echoes=echoes)
 
     def pdhg_init(self, init=None):
         shape = self.vol_dims()

@@ -27,11 +27,11 @@ class TvmeReconstructor(HipPdhgBase):
                  E_total=None, e0=None, e1=None, echo_left_peer=None, echo_right_peer=None,
                  B_total=None, b0=None, b1=None, motion_left_peer=None, motion_right_peer=None,
                  bin_edges=None, world_comm=None, group_comm=None, leader_comm=None, nccl_group=None,
-                 dist_grid=None, tagger=None, show_pbar=True, nufft_backend="hip"):
+                 dist_grid=None, tagger=None, show_pbar=True, nufft_backend="hip", echoes=None):
         self.lamda_m = lamda_m
         self.lamda_e = lamda_e
         self.l2_coupling = bool(l2_coupling)
-        E = int(ksp.shape[0])
+        E = int(ksp.shape[0]) if ksp is not None else int(echoes)
         self.E_total = int(E_total) if E_total is not None else E
         self.e0 = int(e0) if e0 is not None else 0
         self.e1 = int(e1) if e1 is not None else self.e0 + E
@@ -40,7 +40,8 @@ class TvmeReconstructor(HipPdhgBase):
         self._setup(ksp, coord, dcf, mps, resp, dual_q, B_local, margin=margin, device=device,
                     B_total=B_total, b0=b0, b1=b1, bin_edges=bin_edges,
                     comms=(world_comm, group_comm, leader_comm), show_pbar=show_pbar,
-                    nufft_backend=nufft_backend, sigma=sigma, tau=tau, max_iter=max_iter, tol=tol)
+                    nufft_backend=nufft_backend, sigma=sigma, tau=tau, max_iter=max_iter, tol=tol,
+                    echoes=echoes)
 
     def pdhg_init(self, init=None):
         shape = self.vol_dims()

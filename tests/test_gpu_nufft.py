@@ -50,6 +50,9 @@ def test_forward_single_image(torch_mod, shape):
         out = be.forward(img, b).cpu().numpy()
         ref = on.nufft(img.astype(np.complex128), coords[b])
         assert out.shape == ref.shape
+        e = np.abs(out - ref) / np.abs(ref).max()
+        bad = np.nonzero(e > 1e-5)[0]
+        print("bin", b, "bad samples", bad[:20].tolist(), "coords", coords[b][bad[:5]].tolist(), flush=True)
         assert _rel(out, ref) < TOL, (b, _rel(out, ref))
 
 
