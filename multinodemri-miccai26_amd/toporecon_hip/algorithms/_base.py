@@ -79,7 +79,7 @@ class HipPdhgBase:
 
         dev = self.device
         self.maps = self._tensor(mps, torch.complex64)
-        coords, self.y, self.q, self.dcf = [], [], [], []
+        coords, self.y, self.q, self.dcf, self.q0 = [], [], [], [], []
         members = []
         for gb in range(self.b0, self.b1):
             idx = np.nonzero((resp >= bins[gb]) & (resp < bins[gb + 1]))[0]
@@ -100,11 +100,15 @@ class HipPdhgBase:
                           y_s.data_ptr(), stream)
                 del k_b
             q_s = torch.zeros((M, self.K), dtype=torch.complex64, device=dev)
-            if dual_q is not None:
+            q0 = None
+            if dual_q is not None and np.any(np.asarray(dual_q) if not isinstance(dual_q, torch.Tensor)
+                                             else bool(torch.any(dual_q != 0))):
                 q_b = self._take(dual_q, idx, axis=2, dtype=torch.complex64).reshape(self.K, M)
                 _lib.call("tr_plan_gather_samples", self.plan.handle, b, q_b.data_ptr(), self.K,
                           q_s.data_ptr(), stream)
                 del q_b
+                q0 = q_s.clone()
+            self.q0.append(q0)
             d_b = self._take(dcf, idx, axis=0, dtype=torch.float32).reshape(M)
             d_s = torch.empty((M,), dtype=torch.float32, device=dev)
             _lib.call("tr_plan_gather_weights", self.plan.handle, b, d_b.data_ptr(), d_s.data_ptr(),
@@ -154,6 +158,14 @@ class HipPdhgBase:
         return num, den
 
     # -- data term -----------------------------------------------------------------
+    def reset_dual(self):
+        """Data duals back to their constructor value (dual_q, or 0), as at the start of run()."""
+        for b in range(self.B):
+            if self.q0[b] is None:
+                self.q[b].zero_()
+            else:
+                self.q[b].copy_(self.q0[b])
+
     def data_dual_update(self, ubar):
         """q_b <- (q_b + sigma (A ubar_b - y_b)) / (1 + sigma) for every bin (one launch chain per bin)."""
         s = self.stream
@@ -212,6 +224,7 @@ class HipPdhgBase:
         import torch
         with torch.cuda.device(self.device):
             self.pdhg_init(init)
+            self.reset_dual()
             torch.cuda.synchronize(self.device)
             t0 = time.perf_counter()
             self.history = []
