@@ -124,6 +124,29 @@ int tr_tvme_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, c
                    const tr_c64* p_m, const tr_c64* p_ex, const tr_c64* p_ey, const tr_c64* p_ez,
                    float tau, double* partial, void* stream);
 
+/* TVMW motion dual (lambda1) + db1 echo-wavelet dual (lambda2) (tvmw.py:468-520). */
+int tr_tvmw_dual(int B, int E, int Z, int Y, int X, const tr_c64* ubar, tr_c64* p_m, tr_c64* p_w1,
+                 float sigma, float lam1, float lam2, int l2_coupling, void* stream);
+/* TVMW primal: u -= tau ((g - divp_m + W1^H p_w1) + W2^H p_w2); ubar = 2u - u_old; stopping
+ * sums of |ubar_prev - ubar|^2 and |ubar_prev|^2 (tvmw.py:661-684, 723-769). */
+int tr_tvmw_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, const tr_c64* grad,
+                   const tr_c64* p_m, const tr_c64* p_w1, const tr_c64* divw2, float tau, double* partial,
+                   void* stream);
+
+/* ---- db6 zero-mode 3-D wavelet (ptwt.wavedec3 / waverec3, tvmw.py:196-206) ----
+ * Coefficients per image, flat: cA_L, level-L details, ..., level-1 details, each level's
+ * details ordered aad, ada, add, daa, dad, dda, ddd (letters = z, y, x).  `work` holds
+ * nimg * tr_w2_work_count() complex values.  level 0 is the identity. */
+int64_t tr_w2_coeff_count(int Z, int Y, int X, int level);
+int64_t tr_w2_work_count(int Z, int Y, int X, int level);
+int tr_w2_forward(const tr_c64* img, int64_t nimg, int Z, int Y, int X, int level, tr_c64* coeffs,
+                  tr_c64* work, void* stream);
+int tr_w2_adjoint(const tr_c64* coeffs, int64_t nimg, int Z, int Y, int X, int level, tr_c64* out,
+                  tr_c64* work, void* stream);
+/* p <- p + sigma w projected onto |p| <= lam per coefficient with sqrt(re^2 + im^2 + 1e-12)
+ * (0 when lam <= 0) (tvmw.py:162-193). */
+int tr_w2_dual_update(tr_c64* p, const tr_c64* w, int64_t n, float sigma, float lam, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

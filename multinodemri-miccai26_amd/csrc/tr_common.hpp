@@ -61,6 +61,11 @@ enum KernelId : int {
   kKTvmeDual,
   kKTvmePrimal,
   kKReduce,
+  kKW2Forward,
+  kKW2Adjoint,
+  kKW2Dual,
+  kKTvmwDual,
+  kKTvmwPrimal,
   kNumKernelIds
 };
 

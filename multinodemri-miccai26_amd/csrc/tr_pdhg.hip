@@ -221,6 +221,80 @@ __global__ void __launch_bounds__(kEwBlock) k_tvme_primal(VolGeom g, float2* u, 
   block_reduce2(num, den, partial);
 }
 
+// TVMW dual step (tvmw.py:468-520): p_m as TVM (lambda1), then the db1 echo
+// wavelet of ubar with global echo index (even e: (x_e + x_{e+1})/sqrt2 with
+// x_E = 0, odd e: (x_e - x_{e-1})/sqrt2), ascent and p_w1 /= max(1, |p_w1|/lambda2)
+// (p_w1 = 0 when lambda2 <= 0).
+__global__ void __launch_bounds__(kEwBlock) k_tvmw_dual(VolGeom g, const float2* ubar, float2* p_m,
+                                                        float2* p_w1, float sigma, float lam1,
+                                                        float lam2, int l2) {
+  const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
+  if (t >= (int64_t)g.B * g.N) return;
+  const int b = (int)(t / g.N);
+  const int64_t v = t - (int64_t)b * g.N;
+  motion_dual(g, b, v, ubar, p_m, sigma, lam1, l2);
+  const float s = 0.70710678118654752440f;
+  for (int e = 0; e < g.E; ++e) {
+    const int64_t i = ((int64_t)b * g.E + e) * g.N + v;
+    float2 w;
+    if ((e & 1) == 0) {
+      const float2 x1 = (e + 1 < g.E) ? ubar[i + g.N] : make_float2(0.f, 0.f);
+      w = cscale(cadd(ubar[i], x1), s);
+    } else {
+      w = cscale(csub(ubar[i], ubar[i - g.N]), s);
+    }
+    float2 p = p_w1[i];
+    p = make_float2(p.x + sigma * w.x, p.y + sigma * w.y);
+    if (lam2 > 0.f) {
+      const float den = fmaxf(1.0f, sqrtf(cabs2(p)) / lam2);
+      p = make_float2(p.x / den, p.y / den);
+    } else {
+      p = make_float2(0.f, 0.f);
+    }
+    p_w1[i] = p;
+  }
+}
+
+// TVMW primal step (tvmw.py:661-684): divp_m[b] = p[b] - p[b-1], divp_m[0] = p[0];
+// W1^H p_w1 (even e: (a_e - d_{e+1})/sqrt2 with d_E = 0, odd e: (a_{e-1} + d_e)/sqrt2);
+// u <- u - tau ((g - divp_m + divp_w1) + divp_w2); ubar <- 2u - u_old.  Stopping
+// sums (tvmw.py:723-769) in float64: sum |ubar_prev - ubar|^2 and sum |ubar_prev|^2.
+__global__ void __launch_bounds__(kEwBlock) k_tvmw_primal(VolGeom g, float2* u, float2* ubar,
+                                                          const float2* grad, const float2* p_m,
+                                                          const float2* p_w1, const float2* divw2,
+                                                          float tau, double2* partial) {
+  const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
+  double num = 0.0, den = 0.0;
+  if (t < (int64_t)g.B * g.N) {
+    const int b = (int)(t / g.N);
+    const int64_t v = t - (int64_t)b * g.N;
+    const int64_t EN = (int64_t)g.E * g.N;
+    const float s = 0.70710678118654752440f;
+    for (int e = 0; e < g.E; ++e) {
+      const int64_t i = ((int64_t)b * g.E + e) * g.N + v;
+      const float2 dm = (b == 0) ? p_m[i] : csub(p_m[i], p_m[i - EN]);
+      float2 dw1;
+      if ((e & 1) == 0) {
+        const float2 dn = (e + 1 < g.E) ? p_w1[i + g.N] : make_float2(0.f, 0.f);
+        dw1 = cscale(csub(p_w1[i], dn), s);
+      } else {
+        dw1 = cscale(cadd(p_w1[i - g.N], p_w1[i]), s);
+      }
+      const float2 x = cadd(cadd(csub(grad[i], dm), dw1), divw2[i]);
+      const float2 uo = u[i];
+      const float2 un = make_float2(uo.x - tau * x.x, uo.y - tau * x.y);
+      const float2 bo = ubar[i];
+      const float2 bn = make_float2(2.f * un.x - uo.x, 2.f * un.y - uo.y);
+      u[i] = un;
+      ubar[i] = bn;
+      const double ddx = (double)bo.x - (double)bn.x, ddy = (double)bo.y - (double)bn.y;
+      num += ddx * ddx + ddy * ddy;
+      den += (double)bo.x * bo.x + (double)bo.y * bo.y;
+    }
+  }
+  block_reduce2(num, den, partial);
+}
+
 // Fixed-order final reduction of the per-block partial sums (one block).
 __global__ void __launch_bounds__(kEwBlock) k_reduce_partials(const double2* partial, int n, double* out) {
   double a = 0.0, b = 0.0;
@@ -312,6 +386,29 @@ int tr_tvme_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, c
   ProfScope ps(kKTvmePrimal, 8.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
   hipLaunchKernelGGL(k_tvme_primal, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(u),
                      F2(ubar), F2(grad), F2(p_m), F2(p_ex), F2(p_ey), F2(p_ez), tau, (double2*)partial);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_tvmw_dual(int B, int E, int Z, int Y, int X, const tr_c64* ubar, tr_c64* p_m, tr_c64* p_w1,
+                 float sigma, float lam1, float lam2, int l2_coupling, void* stream) {
+  VolGeom g;
+  if (int r = check_geom(B, E, Z, Y, X, &g)) return r;
+  ProfScope ps(kKTvmwDual, 5.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
+  hipLaunchKernelGGL(k_tvmw_dual, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(ubar),
+                     F2(p_m), F2(p_w1), sigma, lam1, lam2, l2_coupling);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_tvmw_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, const tr_c64* grad,
+                   const tr_c64* p_m, const tr_c64* p_w1, const tr_c64* divw2, float tau, double* partial,
+                   void* stream) {
+  VolGeom g;
+  if (int r = check_geom(B, E, Z, Y, X, &g)) return r;
+  ProfScope ps(kKTvmwPrimal, 9.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
+  hipLaunchKernelGGL(k_tvmw_primal, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(u),
+                     F2(ubar), F2(grad), F2(p_m), F2(p_w1), F2(divw2), tau, (double2*)partial);
   TR_LAUNCH_CHECK();
   return kOk;
 }

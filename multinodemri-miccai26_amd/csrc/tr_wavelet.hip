@@ -1,0 +1,350 @@
+// 3-D db6 wavelet of the TVMW spatial regulariser on a batch of complex images.
+//
+// Matches ptwt.wavedec3(x, "db6", mode="zero", level) / ptwt.waverec3 as called
+// at src/toporecon/algorithms/tvmw.py:196-206 (real and imaginary parts see the
+// same real filters): one analysis level is a full convolution with the
+// decomposition filter along z, then y, then x, keeping odd output indices
+// (length n -> (n + 11) / 2); synthesis is its exact transpose.  Coefficients
+// of one image are stored flat in the order cA_L, level-L details, ...,
+// level-1 details, each level's details as aad, ada, add, daa, dad, dda, ddd
+// (letters = z, y, x; a = low pass, d = high pass).
+
+#include "tr_internal.hpp"
+
+namespace tr {
+
+// db6 decomposition filters (Daubechies' 12-tap extremal-phase scaling filter,
+// time-reversed, in PyWavelets' orientation; high pass by the QMF relation).
+__constant__ float c_dec_lo[12] = {
+    -0.0010773010853084792f, 0.0047772575109455064f, 0.0005538422011614997f, -0.031582039317486009f,
+    0.027522865530305706f,   0.097501605587322862f,  -0.12976686756726172f,  -0.22626469396543944f,
+    0.31525035170919768f,    0.75113390802109525f,   0.49462389039845273f,   0.1115407433501094f};
+__constant__ float c_dec_hi[12] = {
+    -0.1115407433501094f,    0.49462389039845273f,   -0.75113390802109525f, 0.31525035170919768f,
+    0.22626469396543944f,    -0.12976686756726172f,  -0.097501605587322862f, 0.027522865530305706f,
+    0.031582039317486009f,   0.0005538422011614997f, -0.0047772575109455064f, -0.0010773010853084792f};
+
+constexpr int kFilt = 12;
+
+struct AxisArgs {
+  int64_t rows;   // images * rows per image
+  int rows_img;   // rows per image
+  int n;          // signal length along the axis
+  int c;          // coefficient length (n + 11) / 2
+  int64_t Q;      // contiguous trailing extent
+  int64_t sig_stride;  // complex elements per image of the signal array
+  int64_t lo_stride, hi_stride;
+  int64_t lo_off, hi_off;
+};
+
+// Analysis along one axis: lo[k] = sum_j h[j] x[2k + 1 - j], zeros outside [0, n).
+__global__ void k_dwt_axis(AxisArgs a, const float2* __restrict__ x, float2* __restrict__ lo,
+                           float2* __restrict__ hi) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = a.rows * a.c * a.Q;
+  if (t >= total) return;
+  const int64_t q = t % a.Q;
+  const int k = (int)((t / a.Q) % a.c);
+  const int64_t row = t / (a.Q * a.c);
+  const int64_t img = row / a.rows_img;
+  const int64_t r = row - img * a.rows_img;
+  const float2* xs = x + img * a.sig_stride + r * a.n * a.Q + q;
+  float2 sl = make_float2(0.f, 0.f), sh = sl;
+#pragma unroll
+  for (int j = 0; j < kFilt; ++j) {
+    const int m = 2 * k + 1 - j;
+    if (m >= 0 && m < a.n) {
+      const float2 v = xs[(int64_t)m * a.Q];
+      sl.x += c_dec_lo[j] * v.x;
+      sl.y += c_dec_lo[j] * v.y;
+      sh.x += c_dec_hi[j] * v.x;
+      sh.y += c_dec_hi[j] * v.y;
+    }
+  }
+  const int64_t o = (r * a.c + k) * a.Q + q;
+  lo[img * a.lo_stride + a.lo_off + o] = sl;
+  hi[img * a.hi_stride + a.hi_off + o] = sh;
+}
+
+// Synthesis along one axis (transpose of k_dwt_axis):
+// x[m] = sum_k lo[k] h_lo[2k + 1 - m] + hi[k] h_hi[2k + 1 - m].
+__global__ void k_idwt_axis(AxisArgs a, const float2* __restrict__ lo, const float2* __restrict__ hi,
+                            float2* __restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = a.rows * a.n * a.Q;
+  if (t >= total) return;
+  const int64_t q = t % a.Q;
+  const int m = (int)((t / a.Q) % a.n);
+  const int64_t row = t / (a.Q * a.n);
+  const int64_t img = row / a.rows_img;
+  const int64_t r = row - img * a.rows_img;
+  const float2* ls = lo + img * a.lo_stride + a.lo_off + r * a.c * a.Q + q;
+  const float2* hs = hi + img * a.hi_stride + a.hi_off + r * a.c * a.Q + q;
+  // taps j = 2k + 1 - m in [0, 11]  <=>  m / 2 <= k <= (m + 10) / 2
+  const int k1 = min(a.c - 1, (m + 10) / 2);
+  float2 s = make_float2(0.f, 0.f);
+  for (int k = m / 2; k <= k1; ++k) {
+    const int j = 2 * k + 1 - m;
+    const float2 vl = ls[(int64_t)k * a.Q];
+    const float2 vh = hs[(int64_t)k * a.Q];
+    s.x += c_dec_lo[j] * vl.x + c_dec_hi[j] * vh.x;
+    s.y += c_dec_lo[j] * vl.y + c_dec_hi[j] * vh.y;
+  }
+  x[img * a.sig_stride + (r * a.n + m) * a.Q + q] = s;
+}
+
+// p <- proj(p + sigma w): t * min(1, lam / sqrt(re^2 + im^2 + 1e-12)); 0 if lam <= 0
+// (tvmw.py:162-193 tree axpy + L-inf projection of every sub-band).
+__global__ void k_w2_dual(float2* p, const float2* w, int64_t n, float sigma, float lam) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (lam <= 0.f) {
+    p[i] = make_float2(0.f, 0.f);
+    return;
+  }
+  const float2 a = p[i], b = w[i];
+  const float2 t = make_float2(a.x + sigma * b.x, a.y + sigma * b.y);
+  const float mag = sqrtf(t.x * t.x + t.y * t.y + 1e-12f);
+  const float s = fminf(lam / mag, 1.0f);
+  p[i] = make_float2(t.x * s, t.y * s);
+}
+
+// ---- host -------------------------------------------------------------------
+struct Level {
+  int n[3];  // input (signal) shape of this level
+  int c[3];  // coefficient shape
+};
+
+static int coeff_len(int n) { return (n + kFilt - 1) / 2; }
+
+static int make_levels(int Z, int Y, int X, int level, std::vector<Level>& lv, int64_t* ncoef,
+                       std::vector<int64_t>& det_off, int64_t* work) {
+  if (level < 0 || level > 8) TR_FAIL(kInvalidArgument, "level must be in [0, 8]");
+  int n[3] = {Z, Y, X};
+  lv.clear();
+  for (int l = 0; l < level; ++l) {
+    Level L;
+    for (int i = 0; i < 3; ++i) {
+      L.n[i] = n[i];
+      L.c[i] = coeff_len(n[i]);
+      n[i] = L.c[i];
+    }
+    lv.push_back(L);
+  }
+  if (level == 0) {
+    *ncoef = (int64_t)Z * Y * X;
+    det_off.clear();
+    *work = 0;
+    return kOk;
+  }
+  const Level& top = lv.back();
+  const int64_t SL = (int64_t)top.c[0] * top.c[1] * top.c[2];
+  // offsets of the detail blocks: level L first, level 1 last
+  det_off.assign(level, 0);
+  int64_t off = SL;
+  for (int l = level - 1; l >= 0; --l) {
+    det_off[l] = off;
+    off += 7 * (int64_t)lv[l].c[0] * lv[l].c[1] * lv[l].c[2];
+  }
+  *ncoef = off;
+  const Level& f = lv[0];
+  const int64_t sz = (int64_t)f.c[0] * f.n[1] * f.n[2];
+  const int64_t sy = (int64_t)f.c[0] * f.c[1] * f.n[2];
+  const int64_t s1 = (int64_t)f.c[0] * f.c[1] * f.c[2];
+  *work = 2 * sz + 4 * sy + 2 * s1;
+  return kOk;
+}
+
+static int launch_axis(const AxisArgs& a, const float2* x, float2* lo, float2* hi, hipStream_t s) {
+  const int64_t total = a.rows * a.c * a.Q;
+  if (total == 0) return kOk;
+  hipLaunchKernelGGL(k_dwt_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, x, lo, hi);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+static int launch_iaxis(const AxisArgs& a, const float2* lo, const float2* hi, float2* x, hipStream_t s) {
+  const int64_t total = a.rows * a.n * a.Q;
+  if (total == 0) return kOk;
+  hipLaunchKernelGGL(k_idwt_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, lo, hi, x);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+static AxisArgs axis(int64_t nimg, int rows_img, int n, int c, int64_t Q, int64_t sig_stride,
+                     int64_t lo_stride, int64_t lo_off, int64_t hi_stride, int64_t hi_off) {
+  AxisArgs a;
+  a.rows = nimg * rows_img;
+  a.rows_img = rows_img;
+  a.n = n;
+  a.c = c;
+  a.Q = Q;
+  a.sig_stride = sig_stride;
+  a.lo_stride = lo_stride;
+  a.hi_stride = hi_stride;
+  a.lo_off = lo_off;
+  a.hi_off = hi_off;
+  return a;
+}
+
+}  // namespace tr
+
+using namespace tr;
+
+extern "C" {
+
+int64_t tr_w2_coeff_count(int Z, int Y, int X, int level) {
+  std::vector<Level> lv;
+  std::vector<int64_t> off;
+  int64_t n = -1, w = 0;
+  if (make_levels(Z, Y, X, level, lv, &n, off, &w)) return -1;
+  return n;
+}
+
+int64_t tr_w2_work_count(int Z, int Y, int X, int level) {
+  std::vector<Level> lv;
+  std::vector<int64_t> off;
+  int64_t n = 0, w = -1;
+  if (make_levels(Z, Y, X, level, lv, &n, off, &w)) return -1;
+  return w;
+}
+
+int tr_w2_forward(const tr_c64* img, int64_t nimg, int Z, int Y, int X, int level, tr_c64* coeffs,
+                  tr_c64* work, void* stream) {
+  std::vector<Level> lv;
+  std::vector<int64_t> det;
+  int64_t ncoef = 0, wn = 0;
+  if (int r = make_levels(Z, Y, X, level, lv, &ncoef, det, &wn)) return r;
+  hipStream_t s = (hipStream_t)stream;
+  ProfScope ps(kKW2Forward, 2.0 * 8.0 * nimg * (double)Z * Y * X + 8.0 * nimg * ncoef, s);
+  if (level == 0) {
+    TR_HIP(hipMemcpyAsync(coeffs, img, sizeof(float2) * nimg * ncoef, hipMemcpyDeviceToDevice, s));
+    return kOk;
+  }
+  const Level& f = lv[0];
+  const int64_t sz = (int64_t)f.c[0] * f.n[1] * f.n[2];
+  const int64_t sy = (int64_t)f.c[0] * f.c[1] * f.n[2];
+  const int64_t s1 = (int64_t)f.c[0] * f.c[1] * f.c[2];
+  float2* W = F2(work);
+  float2* zl = W;                        // nimg * sz each (per-level compact layouts)
+  float2* zh = zl + nimg * sz;
+  float2* yb[4] = {zh + nimg * sz, nullptr, nullptr, nullptr};
+  for (int i = 1; i < 4; ++i) yb[i] = yb[i - 1] + nimg * sy;
+  float2* abuf[2] = {yb[3] + nimg * sy, yb[3] + nimg * sy + nimg * s1};
+  float2* C = F2(coeffs);
+  const float2* a_in = F2(img);
+  int64_t a_stride = (int64_t)Z * Y * X;
+  for (int l = 0; l < level; ++l) {
+    const Level& L = lv[l];
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+    const int c0 = L.c[0], c1 = L.c[1], c2 = L.c[2];
+    const int64_t lz = (int64_t)c0 * n1 * n2, ly = (int64_t)c0 * c1 * n2, lc = (int64_t)c0 * c1 * c2;
+    // z
+    if (int r = launch_axis(axis(nimg, 1, n0, c0, (int64_t)n1 * n2, a_stride, lz, 0, lz, 0), a_in, zl, zh, s))
+      return r;
+    // y
+    if (int r = launch_axis(axis(nimg, c0, n1, c1, n2, lz, ly, 0, ly, 0), zl, yb[0], yb[1], s)) return r;
+    if (int r = launch_axis(axis(nimg, c0, n1, c1, n2, lz, ly, 0, ly, 0), zh, yb[2], yb[3], s)) return r;
+    // x: LL -> (aaa, aad), LH -> (ada, add), HL -> (daa, dad), HH -> (dda, ddd)
+    const bool last = (l == level - 1);
+    float2* a_out = last ? C : abuf[l & 1];
+    const int64_t a_out_stride = last ? ncoef : lc;
+    const int64_t d = det[l];
+    if (int r = launch_axis(axis(nimg, c0 * c1, n2, c2, 1, ly, a_out_stride, 0, ncoef, d + 0 * lc), yb[0],
+                            a_out, C, s))
+      return r;
+    if (int r = launch_axis(axis(nimg, c0 * c1, n2, c2, 1, ly, ncoef, d + 1 * lc, ncoef, d + 2 * lc), yb[1],
+                            C, C, s))
+      return r;
+    if (int r = launch_axis(axis(nimg, c0 * c1, n2, c2, 1, ly, ncoef, d + 3 * lc, ncoef, d + 4 * lc), yb[2],
+                            C, C, s))
+      return r;
+    if (int r = launch_axis(axis(nimg, c0 * c1, n2, c2, 1, ly, ncoef, d + 5 * lc, ncoef, d + 6 * lc), yb[3],
+                            C, C, s))
+      return r;
+    a_in = a_out;
+    a_stride = lc;
+  }
+  (void)sz;
+  (void)sy;
+  (void)s1;
+  return kOk;
+}
+
+int tr_w2_adjoint(const tr_c64* coeffs, int64_t nimg, int Z, int Y, int X, int level, tr_c64* out,
+                  tr_c64* work, void* stream) {
+  std::vector<Level> lv;
+  std::vector<int64_t> det;
+  int64_t ncoef = 0, wn = 0;
+  if (int r = make_levels(Z, Y, X, level, lv, &ncoef, det, &wn)) return r;
+  hipStream_t s = (hipStream_t)stream;
+  ProfScope ps(kKW2Adjoint, 2.0 * 8.0 * nimg * (double)Z * Y * X + 8.0 * nimg * ncoef, s);
+  if (level == 0) {
+    TR_HIP(hipMemcpyAsync(out, coeffs, sizeof(float2) * nimg * ncoef, hipMemcpyDeviceToDevice, s));
+    return kOk;
+  }
+  const Level& f = lv[0];
+  const int64_t sz = (int64_t)f.c[0] * f.n[1] * f.n[2];
+  const int64_t sy = (int64_t)f.c[0] * f.c[1] * f.n[2];
+  const int64_t s1 = (int64_t)f.c[0] * f.c[1] * f.c[2];
+  float2* W = F2(work);
+  float2* zl = W;
+  float2* zh = zl + nimg * sz;
+  float2* yb[4] = {zh + nimg * sz, nullptr, nullptr, nullptr};
+  for (int i = 1; i < 4; ++i) yb[i] = yb[i - 1] + nimg * sy;
+  float2* abuf[2] = {yb[3] + nimg * sy, yb[3] + nimg * sy + nimg * s1};
+  const float2* C = F2(coeffs);
+  // a_L lives in the coefficient buffer (offset 0, stride ncoef)
+  const float2* a_in = C;
+  int64_t a_in_stride = ncoef;
+  for (int l = level - 1; l >= 0; --l) {
+    const Level& L = lv[l];
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+    const int c0 = L.c[0], c1 = L.c[1], c2 = L.c[2];
+    const int64_t lz = (int64_t)c0 * n1 * n2, ly = (int64_t)c0 * c1 * n2, lc = (int64_t)c0 * c1 * c2;
+    const int64_t d = det[l];
+    // x: (aaa, aad) -> LL, (ada, add) -> LH, (daa, dad) -> HL, (dda, ddd) -> HH
+    // The aaa input has its own stride, so LL is assembled with an a-stride axis call.
+    {
+      AxisArgs a = axis(nimg, c0 * c1, n2, c2, 1, ly, a_in_stride, 0, ncoef, d + 0 * lc);
+      if (int r = launch_iaxis(a, a_in, C, yb[0], s)) return r;
+    }
+    if (int r = launch_iaxis(axis(nimg, c0 * c1, n2, c2, 1, ly, ncoef, d + 1 * lc, ncoef, d + 2 * lc), C, C,
+                             yb[1], s))
+      return r;
+    if (int r = launch_iaxis(axis(nimg, c0 * c1, n2, c2, 1, ly, ncoef, d + 3 * lc, ncoef, d + 4 * lc), C, C,
+                             yb[2], s))
+      return r;
+    if (int r = launch_iaxis(axis(nimg, c0 * c1, n2, c2, 1, ly, ncoef, d + 5 * lc, ncoef, d + 6 * lc), C, C,
+                             yb[3], s))
+      return r;
+    // y: (LL, LH) -> L, (HL, HH) -> H
+    if (int r = launch_iaxis(axis(nimg, c0, n1, c1, n2, lz, ly, 0, ly, 0), yb[0], yb[1], zl, s)) return r;
+    if (int r = launch_iaxis(axis(nimg, c0, n1, c1, n2, lz, ly, 0, ly, 0), yb[2], yb[3], zh, s)) return r;
+    // z: (L, H) -> a_{l-1}
+    const bool first = (l == 0);
+    float2* a_out = first ? F2(out) : abuf[l & 1];
+    const int64_t a_out_stride = first ? (int64_t)Z * Y * X : (int64_t)n0 * n1 * n2;
+    if (int r = launch_iaxis(axis(nimg, 1, n0, c0, (int64_t)n1 * n2, a_out_stride, lz, 0, lz, 0), zl, zh,
+                             a_out, s))
+      return r;
+    a_in = a_out;
+    a_in_stride = a_out_stride;
+  }
+  (void)sz;
+  (void)sy;
+  (void)s1;
+  return kOk;
+}
+
+int tr_w2_dual_update(tr_c64* p, const tr_c64* w, int64_t n, float sigma, float lam, void* stream) {
+  if (n <= 0) return kOk;
+  hipStream_t s = (hipStream_t)stream;
+  ProfScope ps(kKW2Dual, 24.0 * n, s);
+  hipLaunchKernelGGL(k_w2_dual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, F2(p), F2(w), n, sigma,
+                     lam);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // extern "C"

@@ -43,3 +43,22 @@ def test_backend_registry_errors():
         create_nufft_backend("cufinufft", img_shape=(4, 4, 4), coordinates=[], device=0)
     with pytest.raises(RuntimeError, match="SigPy is required"):
         create_nufft_backend(" SigPy ", img_shape=(4, 4, 4), coordinates=[], device=0)
+
+
+def test_db6_constants_match_derived_filters():
+    """The db6 taps hard-coded in the HIP source equal Daubechies' construction (oracle)."""
+    import numpy as np
+
+    from oracle import wavelet as ow
+    src = open(os.path.join(REPO, "multinodemri-miccai26_amd", "csrc", "tr_wavelet.hip")).read()
+    for name, ref in zip(("c_dec_lo", "c_dec_hi"), ow.filters()):
+        body = src.split(name + "[12] = {", 1)[1].split("}", 1)[0]
+        vals = np.array([float(v.strip().rstrip("f")) for v in body.split(",")])
+        assert np.abs(vals - ref).max() < 1e-16
+
+
+def test_db6_level_rule():
+    from oracle import wavelet as ow
+    from toporecon_hip.algorithms.tvmw import db6_max_level
+    for shape in [(80, 192, 192), (32, 64, 64), (21, 40, 40), (22, 22, 22), (44, 200, 90), (10, 12, 12)]:
+        assert db6_max_level(shape) == ow.max_level(shape)
