@@ -230,3 +230,59 @@ def simulate_kspace_numpy(acq: Acquisition, forward) -> np.ndarray:
     rng = np.random.RandomState(cfg.seed + 104729)
     ksp += (rng.standard_normal(ksp.shape) * sig + 1j * rng.standard_normal(ksp.shape) * sig).astype(np.complex64)
     return cond.normalize_kspace(ksp)
+
+
+def write_raw_dataset(acq: "Acquisition", ksp_full, coord_full, dens_full, resp_full, root,
+                      voxel_size=(1.0, 1.0, 1.0)):
+    """Write a raw MICCAI-style dataset (+ prepared mps/resp) for the reference CLI layout.
+
+    ksp_full (E, C, Ntr, Nro) complex; coord_full (Ntr, Nro, 3) in [z, y, x] SigPy units;
+    written as ktraj in [x, y, z] order divided by voxel * n (the main multiplies back,
+    tvme.py:778-783); imageDim.txt / voxelSize.txt as "X:Y:Z".
+    """
+    import os
+
+    from .data import cfl
+    os.makedirs(root, exist_ok=True)
+    Z, Y, X = acq.cfg.img_shape
+    E, C, T, R = ksp_full.shape
+    cfl.write_cfl(os.path.join(root, "ksp"), np.asarray(ksp_full).reshape(1, 1, 1, E, 1, C, T, R, 1))
+    vz = np.asarray(voxel_size, dtype=np.float64)
+    scale = vz * np.array([Z, Y, X], dtype=np.float64)
+    raw = (np.asarray(coord_full, dtype=np.float64) / scale)[..., ::-1].astype(np.float32)
+    cfl.write_cfl(os.path.join(root, "ktraj"), raw.reshape((1, 1, 1, 1, 1, 1, T, R, 3)))
+    cfl.write_cfl(os.path.join(root, "dens"), np.asarray(dens_full).reshape(1, 1, 1, 1, 1, 1, T, R, 1))
+    with open(os.path.join(root, "imageDim.txt"), "w") as f:
+        f.write(f"{X}:{Y}:{Z}\n")
+    with open(os.path.join(root, "voxelSize.txt"), "w") as f:
+        f.write(":".join(str(v) for v in vz[::-1]) + "\n")
+    with open(os.path.join(root, "tr.txt"), "w") as f:
+        f.write(f"{acq.cfg.tr_s * 1000.0}\n")
+    cfl.write_cfl(os.path.join(root, "mps"), acq.mps.reshape((1, 1, 1, 1, 1, C, Z, Y, X)))
+    cfl.write_cfl(os.path.join(root, "resp"), np.asarray(resp_full).reshape(-1, 1))
+
+
+def make_raw(cfg: AcquisitionConfig, forward, noise_seed=None):
+    """Unconditioned raw arrays of a synthetic acquisition, as stored in a raw dataset.
+
+    Returns (acq, ksp_full (E, C, Ntr, Nro), coord_full, dens_full, resp_full); every
+    interleave is simulated with the motion state of its respiratory value.
+    ``forward(images (E, C, Z, Y, X), coord (M, 3)) -> (E, C, M)``.
+    """
+    acq = make_acquisition(cfg)
+    coord_full = cone_trajectory(cfg.interleaves, cfg.readouts, cfg.img_shape, cfg.turns, cfg.cone_angle)
+    dens_full = density(coord_full)
+    resp_full = respiratory_trace(cfg.interleaves, cfg.tr_s, cfg.resp_period_s)
+    sim = np.clip(np.searchsorted(acq.bin_edges, resp_full, side="right") - 1, 0, cfg.bins - 1)
+    E, C = cfg.echoes, cfg.coils
+    ksp = np.zeros((E, C, cfg.interleaves, cfg.readouts), dtype=np.complex64)
+    for b in range(cfg.bins):
+        idx = np.nonzero(sim == b)[0]
+        if idx.size:
+            y = forward(acq.truth[b][:, None] * acq.mps[None], coord_full[idx].reshape(-1, 3))
+            ksp[:, :, idx] = np.asarray(y).reshape(E, C, idx.size, cfg.readouts)
+    rms = float(np.sqrt(np.mean(np.abs(ksp) ** 2)))
+    sig = noise_sigma(rms, cfg.snr_db) / math.sqrt(2.0)
+    rng = np.random.RandomState(cfg.seed + 104729 if noise_seed is None else noise_seed)
+    ksp += (rng.standard_normal(ksp.shape) * sig + 1j * rng.standard_normal(ksp.shape) * sig).astype(np.complex64)
+    return acq, ksp, coord_full, dens_full, resp_full
