@@ -23,23 +23,48 @@ namespace tr {
 constexpr int kWidth = 4;
 constexpr int kTile = 8;  // sample sort tile edge (grid points)
 constexpr int kMaxKc = 16;  // channels per FFT workgroup
-constexpr int kSpreadSeg = 128;  // CSR entries per heavy-row segment
+constexpr int kSpreadSeg = 256;  // samples per heavy-cluster segment
+
+// Per-sample taps (sorted order): natural positions of the first tap per axis,
+// tap counts (4 or 5) and float32 Kaiser-Bessel weights (x weights already
+// divided by width^3), evaluated once at plan time.
+struct __align__(16) TapRec {
+  int p0[3];   // (t0 - g//2) mod g per axis (z, y, x)
+  int n;       // nz | ny << 4 | nx << 8
+  float w[16];  // z: 0..4, y: 5..9, x: 10..14
+};
+
+// The adjoint gathers over clusters of kClZ x kClY x kClX grid points: a wave
+// keeps the cluster's sums for 64 channels in registers and walks the sorted
+// samples touching the cluster once.
+constexpr int kClZ = 2, kClY = 4, kClX = 4;
+constexpr int kClPts = kClZ * kClY * kClX;
+
+// One (cluster, sample) pair: the sample's sorted index and its weights on the
+// cluster's points per axis (zero off the taps or past the grid edge).
+struct __align__(16) ClusterPair {
+  int m;
+  float wz[kClZ];
+  float wy[kClY];
+  float wx[kClX];  // includes 1 / width^3
+  int pad;
+};
 
 struct BinData {
   int64_t M = 0;
-  int64_t nnz = 0;
+  int64_t nnz = 0;          // (cluster, sample) pairs
   int* perm = nullptr;      // sorted position -> original sample index
   float4* kp = nullptr;     // sorted scaled coordinates (kz', ky', kx', 0)
-  int* row_ptr = nullptr;   // G + 1, over FFT-natural grid index
-  int* col = nullptr;       // nnz sorted-sample indices
-  float* val = nullptr;     // nnz weights (already divided by width^3)
-  // rows longer than kSpreadSeg, cut into segments of kSpreadSeg entries
+  TapRec* tap = nullptr;    // sorted per-sample taps
+  int* row_ptr = nullptr;   // NCL + 1, over clusters
+  ClusterPair* pairs = nullptr;  // nnz, sample order within a cluster
+  // clusters with more than kSpreadSeg samples, cut into segments of kSpreadSeg
   int64_t nseg = 0;
   int64_t nheavy = 0;
-  int* seg_row = nullptr;
-  int* seg_j0 = nullptr;
-  int* heavy_row = nullptr;
-  int* heavy_seg_ptr = nullptr;
+  int* seg_row = nullptr;        // cluster of each segment
+  int* seg_j0 = nullptr;         // first pair of each segment
+  int* heavy_row = nullptr;      // heavy cluster ids
+  int* heavy_seg_ptr = nullptr;  // nheavy + 1
 };
 
 }  // namespace tr
@@ -58,8 +83,10 @@ struct tr_plan {
   std::vector<tr::BinData> bins;
   float2* work = nullptr;  // grid workspace, G * work_k complex
   int64_t work_k = 0;
-  float2* partial = nullptr;  // heavy-row segment partial sums
+  float2* partial = nullptr;  // heavy-cluster segment partial sums
   int64_t partial_n = 0;
+  int ncl[3];                 // clusters per axis
+  int64_t NCL;
 };
 
 namespace tr {
@@ -125,6 +152,7 @@ struct InterpArgs {
   GridGeom gg;
   float beta;
   const float4* kp;
+  const TapRec* tap;
   int64_t M;
   int K;
   const float2* grid;
@@ -140,12 +168,13 @@ struct InterpArgs {
 };
 
 struct SpreadArgs {
-  int64_t G;
+  GridGeom gg;
+  int ncl[3];
+  int64_t NCL;
   int K;
   int64_t M;
   const int* row_ptr;
-  const int* col;
-  const float* val;
+  const ClusterPair* pairs;
   int sorted_src;  // 1: q (sorted, [M][K]); 0: ksp (original order, [K][M])
   // sorted source: q[m * K + k] (optionally times dcf_sorted[m])
   const float2* q;
@@ -155,14 +184,15 @@ struct SpreadArgs {
   const float* dcf_orig;
   const int* perm;
   float2* grid;
-  // heavy-row segments
+  // heavy-cluster segments
   int64_t nseg;
   const int* seg_row;
   const int* seg_j0;
   int64_t nheavy;
   const int* heavy_row;
   const int* heavy_seg_ptr;
-  float2* partial;  // nseg * K
+  float2* partial;  // nseg * kClPts * K
+  int ngrp;         // channel groups per cluster
 };
 
 // Loads of a line tile are staged in registers (all of a thread's loads are

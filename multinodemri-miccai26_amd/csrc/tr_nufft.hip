@@ -184,12 +184,8 @@ int tr_nufft_adjoint(tr_plan* p, int bin, const tr_c64* ksp, const float* dcf, c
   const int K = E * C;
   if (int r = ensure_work(p, K)) return r;
   SpreadArgs a = {};
-  a.G = p->G;
   a.K = K;
   a.M = b.M;
-  a.row_ptr = b.row_ptr;
-  a.col = b.col;
-  a.val = b.val;
   a.sorted_src = 0;
   a.ksp = F2(ksp);
   a.dcf_orig = dcf;
@@ -269,12 +265,8 @@ int tr_data_adjoint(tr_plan* p, int bin, const tr_c64* q_sorted, const float* dc
   const int K = E * C;
   if (int r = ensure_work(p, K)) return r;
   SpreadArgs a = {};
-  a.G = p->G;
   a.K = K;
   a.M = b.M;
-  a.row_ptr = b.row_ptr;
-  a.col = b.col;
-  a.val = b.val;
   a.sorted_src = 1;
   a.q = F2(q_sorted);
   a.dcf_sorted = dcf_sorted;

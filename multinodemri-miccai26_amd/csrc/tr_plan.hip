@@ -1,5 +1,6 @@
 // NUFFT plan: coordinate scaling (SigPy float32 semantics), tile sort of the
-// samples, the deterministic spreading CSR, tables and workspace management.
+// samples, per-sample taps, the (cluster, sample) lists of the deterministic
+// adjoint, tables and workspace management.
 
 #include <hipcub/hipcub.hpp>
 
@@ -41,53 +42,101 @@ __global__ void k_gather_kp(const float4* kp_in, const int* perm, int64_t M, flo
   if (j < M) kp_out[j] = kp_in[perm[j]];
 }
 
-__device__ __forceinline__ int tap_count(float k) {
-  if (!isfinite(k) || fabsf(k) > 1.0e6f) return 0;  // same rule as axis_taps
-  const double kd = (double)k;
-  return (int)floor(kd + 2.0) - (int)ceil(kd - 2.0) + 1;
-}
-
-__global__ void k_count_taps(const float4* kp, int64_t M, int* cnt) {
+// Per-sample taps in sorted order; cnt[j] = number of clusters the sample's
+// taps touch.
+__global__ void k_tap_records(const float4* kp, int64_t M, int3 g, float beta, TapRec* tap, int* cnt) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   const float4 k = kp[j];
-  cnt[j] = tap_count(k.x) * tap_count(k.y) * tap_count(k.z);
+  const float kk[3] = {k.x, k.y, k.z};
+  const int gg[3] = {g.x, g.y, g.z};
+  const int csz[3] = {kClZ, kClY, kClX};
+  TapRec r;
+  int count = 1;
+  int packed = 0;
+  for (int ax = 0; ax < 3; ++ax) {
+    float w[5];
+    int pos[5];
+    const int n = axis_taps(kk[ax], gg[ax], beta, ax == 2 ? 1.0f / (kWidth * kWidth * kWidth) : 1.0f, w, pos);
+    r.p0[ax] = pos[0];
+    packed |= n << (4 * ax);
+    for (int d = 0; d < 5; ++d) r.w[5 * ax + d] = w[d];
+    // taps sit at consecutive positions (mod g): count the distinct clusters
+    int cl[5];
+    int nc = 0;
+    for (int d = 0; d < n; ++d) {
+      const int c = pos[d] / csz[ax];
+      bool seen = false;
+      for (int e = 0; e < nc; ++e) seen |= cl[e] == c;
+      if (!seen) cl[nc++] = c;
+    }
+    count *= nc;
+  }
+  r.w[15] = 0.0f;
+  r.n = packed;
+  tap[j] = r;
+  cnt[j] = count;
 }
 
-__global__ void k_fill_pairs(const float4* kp, int64_t M, const int* off, int3 g, float beta,
-                             unsigned* key, unsigned long long* val) {
+// Distinct clusters touched along one axis, with the weight of every cluster
+// point (a point hit by several taps, possible only when g < 5, gets their sum).
+template <int CS>
+__device__ int axis_clusters(const TapRec& r, int ax, int g, int cl[5], float cw[5][CS]) {
+  const int n = (r.n >> (4 * ax)) & 15;
+  const int p0 = r.p0[ax];
+  int nc = 0;
+  for (int d = 0; d < n; ++d) {
+    const int c = ((p0 + d) % g) / CS;
+    bool seen = false;
+    for (int e = 0; e < nc; ++e) seen |= cl[e] == c;
+    if (!seen) cl[nc++] = c;
+  }
+  for (int e = 0; e < nc; ++e)
+    for (int i = 0; i < CS; ++i) {
+      const int q = cl[e] * CS + i;
+      float sum = 0.0f;
+      for (int d = 0; d < n; ++d)
+        if ((p0 + d) % g == q) sum += r.w[5 * ax + d];
+      cw[e][i] = sum;
+    }
+  return nc;
+}
+
+__global__ void k_fill_cluster_pairs(const TapRec* tap, int64_t M, const int* off, int3 g, int3 ncl,
+                                     unsigned* key, int* idx, ClusterPair* rec) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
-  const float4 k = kp[j];
-  float wz[5], wy[5], wx[5];
-  int pz[5], py[5], px[5];
-  const int nz = axis_taps(k.x, g.x, beta, 1.0f, wz, pz);
-  const int ny = axis_taps(k.y, g.y, beta, 1.0f, wy, py);
-  const int nx = axis_taps(k.z, g.z, beta, 1.0f / (kWidth * kWidth * kWidth), wx, px);
+  const TapRec r = tap[j];
+  int cz[5], cy[5], cx[5];
+  float wz[5][kClZ], wy[5][kClY], wx[5][kClX];
+  const int nz = axis_clusters<kClZ>(r, 0, g.x, cz, wz);
+  const int ny = axis_clusters<kClY>(r, 1, g.y, cy, wy);
+  const int nx = axis_clusters<kClX>(r, 2, g.z, cx, wx);
   int p = off[j];
-  for (int dz = 0; dz < nz; ++dz)
-    for (int dy = 0; dy < ny; ++dy) {
-      const float wzy = wz[dz] * wy[dy];
-      for (int dx = 0; dx < nx; ++dx) {
-        const float w = wzy * wx[dx];
-        key[p] = (unsigned)(((int64_t)pz[dz] * g.y + py[dy]) * g.z + px[dx]);
-        val[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned long long)__float_as_uint(w);
+  for (int a = 0; a < nz; ++a)
+    for (int b = 0; b < ny; ++b)
+      for (int c = 0; c < nx; ++c) {
+        ClusterPair o;
+        o.m = (int)j;
+        for (int i = 0; i < kClZ; ++i) o.wz[i] = wz[a][i];
+        for (int i = 0; i < kClY; ++i) o.wy[i] = wy[b][i];
+        for (int i = 0; i < kClX; ++i) o.wx[i] = wx[c][i];
+        o.pad = 0;
+        key[p] = (unsigned)(((int64_t)cz[a] * ncl.y + cy[b]) * ncl.z + cx[c]);
+        idx[p] = p;
+        rec[p] = o;
         ++p;
       }
-    }
 }
 
-__global__ void k_unpack_pairs(const unsigned long long* val, int64_t nnz, int* col, float* w) {
+__global__ void k_gather_pairs(const ClusterPair* in, const int* idx, int64_t nnz, ClusterPair* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnz) return;
-  const unsigned long long v = val[i];
-  col[i] = (int)(v >> 32);
-  w[i] = __uint_as_float((unsigned)(v & 0xffffffffull));
+  if (i < nnz) out[i] = in[idx[i]];
 }
 
-__global__ void k_row_ptr(const unsigned* key, int64_t nnz, int64_t G, int* row_ptr) {
+__global__ void k_row_ptr(const unsigned* key, int64_t nnz, int64_t R, int* row_ptr) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > G) return;
+  if (t > R) return;
   // lower_bound(key, t)
   int64_t lo = 0, hi = nnz;
   while (lo < hi) {
@@ -145,12 +194,13 @@ int ensure_work(tr_plan* p, int64_t K) {
 
 int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s) {
   b.M = M;
-  TR_HIP(hipMalloc(&b.row_ptr, sizeof(int) * (p->G + 1)));
+  TR_HIP(hipMalloc(&b.row_ptr, sizeof(int) * (p->NCL + 1)));
   if (M == 0) {
-    TR_HIP(hipMemsetAsync(b.row_ptr, 0, sizeof(int) * (p->G + 1), s));
+    TR_HIP(hipMemsetAsync(b.row_ptr, 0, sizeof(int) * (p->NCL + 1), s));
+    TR_HIP(hipStreamSynchronize(s));
     return kOk;
   }
-  // 125 taps per sample must index an int32 CSR.
+  // up to 4 x 3 x 3 clusters per sample must index an int32 pair list
   if (M > (int64_t)17000000) TR_FAIL(kUnsupported, "more than 17M samples in one bin");
   float4* kp_raw = nullptr;
   unsigned *key_in = nullptr, *key_out = nullptr;
@@ -161,11 +211,13 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   TR_HIP(hipMalloc(&idx, sizeof(int) * M));
   TR_HIP(hipMalloc(&b.perm, sizeof(int) * M));
   TR_HIP(hipMalloc(&b.kp, sizeof(float4) * M));
+  TR_HIP(hipMalloc(&b.tap, sizeof(TapRec) * M));
   int3 nt;
   nt.x = p->g[0] / kTile + 2;
   nt.y = p->g[1] / kTile + 2;
   nt.z = p->g[2] / kTile + 2;
   const int3 g3 = make_int3(p->g[0], p->g[1], p->g[2]);
+  const int3 ncl3 = make_int3(p->ncl[0], p->ncl[1], p->ncl[2]);
   hipLaunchKernelGGL(k_scale_keys, dim3(grid1d(M, 256)), dim3(256), 0, s, coord, M,
                      make_float3(p->coord_scale[0], p->coord_scale[1], p->coord_scale[2]),
                      make_float3(p->coord_shift[0], p->coord_shift[1], p->coord_shift[2]), g3, nt,
@@ -182,13 +234,13 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
                                             key_bits, s));
   hipLaunchKernelGGL(k_gather_kp, dim3(grid1d(M, 256)), dim3(256), 0, s, kp_raw, b.perm, M, b.kp);
   TR_LAUNCH_CHECK();
-  // tap counts -> offsets
+  // taps and cluster counts -> pair offsets
   int* cnt = nullptr;
   int* off = nullptr;
   TR_HIP(hipMalloc(&cnt, sizeof(int) * (M + 1)));
   TR_HIP(hipMalloc(&off, sizeof(int) * (M + 1)));
   TR_HIP(hipMemsetAsync(cnt + M, 0, sizeof(int), s));
-  hipLaunchKernelGGL(k_count_taps, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, cnt);
+  hipLaunchKernelGGL(k_tap_records, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, g3, p->beta, b.tap, cnt);
   TR_LAUNCH_CHECK();
   size_t scan_bytes = 0;
   TR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, (int)(M + 1), s));
@@ -201,44 +253,45 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   const int64_t nnz = nnz_i;
   b.nnz = nnz;
   if (nnz == 0) {
-    TR_HIP(hipMemsetAsync(b.row_ptr, 0, sizeof(int) * (p->G + 1), s));
+    TR_HIP(hipMemsetAsync(b.row_ptr, 0, sizeof(int) * (p->NCL + 1), s));
     TR_HIP(hipStreamSynchronize(s));
     void* frees0[] = {kp_raw, key_in, key_out, idx, tmp, cnt, off, scan_tmp};
     for (void* f : frees0) TR_HIP(hipFree(f));
     return kOk;
   }
   unsigned *pk_in = nullptr, *pk_out = nullptr;
-  unsigned long long *pv_in = nullptr, *pv_out = nullptr;
+  int *pi_in = nullptr, *pi_out = nullptr;
+  ClusterPair* rec = nullptr;
   TR_HIP(hipMalloc(&pk_in, sizeof(unsigned) * nnz));
   TR_HIP(hipMalloc(&pk_out, sizeof(unsigned) * nnz));
-  TR_HIP(hipMalloc(&pv_in, sizeof(unsigned long long) * nnz));
-  TR_HIP(hipMalloc(&pv_out, sizeof(unsigned long long) * nnz));
-  hipLaunchKernelGGL(k_fill_pairs, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, off, g3, p->beta,
-                     pk_in, pv_in);
+  TR_HIP(hipMalloc(&pi_in, sizeof(int) * nnz));
+  TR_HIP(hipMalloc(&pi_out, sizeof(int) * nnz));
+  TR_HIP(hipMalloc(&rec, sizeof(ClusterPair) * nnz));
+  hipLaunchKernelGGL(k_fill_cluster_pairs, dim3(grid1d(M, 256)), dim3(256), 0, s, b.tap, M, off, g3, ncl3,
+                     pk_in, pi_in, rec);
   TR_LAUNCH_CHECK();
-  int gbits = 1;
-  while (((int64_t)1 << gbits) < p->G) ++gbits;
+  int cbits = 1;
+  while (((int64_t)1 << cbits) < p->NCL) ++cbits;
+  // radix sort is stable: within a cluster the pairs keep increasing sample order
   size_t tmp2_bytes = 0;
-  TR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2_bytes, pk_in, pk_out, pv_in, pv_out,
-                                            (int)nnz, 0, gbits, s));
+  TR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2_bytes, pk_in, pk_out, pi_in, pi_out, (int)nnz,
+                                            0, cbits, s));
   void* tmp2 = nullptr;
   TR_HIP(hipMalloc(&tmp2, tmp2_bytes));
-  TR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp2, tmp2_bytes, pk_in, pk_out, pv_in, pv_out, (int)nnz,
-                                            0, gbits, s));
-  TR_HIP(hipMalloc(&b.col, sizeof(int) * nnz));
-  TR_HIP(hipMalloc(&b.val, sizeof(float) * nnz));
-  hipLaunchKernelGGL(k_unpack_pairs, dim3(grid1d(nnz, 256)), dim3(256), 0, s, pv_out, nnz, b.col,
-                     b.val);
+  TR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp2, tmp2_bytes, pk_in, pk_out, pi_in, pi_out, (int)nnz, 0,
+                                            cbits, s));
+  TR_HIP(hipMalloc(&b.pairs, sizeof(ClusterPair) * nnz));
+  hipLaunchKernelGGL(k_gather_pairs, dim3(grid1d(nnz, 256)), dim3(256), 0, s, rec, pi_out, nnz, b.pairs);
   TR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_row_ptr, dim3(grid1d(p->G + 1, 256)), dim3(256), 0, s, pk_out, nnz, p->G,
+  hipLaunchKernelGGL(k_row_ptr, dim3(grid1d(p->NCL + 1, 256)), dim3(256), 0, s, pk_out, nnz, p->NCL,
                      b.row_ptr);
   TR_LAUNCH_CHECK();
   TR_HIP(hipStreamSynchronize(s));
-  {  // heavy rows -> fixed segments of kSpreadSeg entries
-    std::vector<int> rp(p->G + 1);
-    TR_HIP(hipMemcpy(rp.data(), b.row_ptr, sizeof(int) * (p->G + 1), hipMemcpyDeviceToHost));
+  {  // heavy clusters -> fixed segments of kSpreadSeg pairs
+    std::vector<int> rp(p->NCL + 1);
+    TR_HIP(hipMemcpy(rp.data(), b.row_ptr, sizeof(int) * (p->NCL + 1), hipMemcpyDeviceToHost));
     std::vector<int> seg_row, seg_j0, heavy_row, heavy_ptr;
-    for (int64_t t = 0; t < p->G; ++t) {
+    for (int64_t t = 0; t < p->NCL; ++t) {
       const int len = rp[t + 1] - rp[t];
       if (len <= kSpreadSeg) continue;
       heavy_row.push_back((int)t);
@@ -263,7 +316,7 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
                        hipMemcpyHostToDevice));
     }
   }
-  void* frees[] = {kp_raw, key_in, key_out, idx, tmp, cnt, off, scan_tmp, pk_in, pk_out, pv_in, pv_out, tmp2};
+  void* frees[] = {kp_raw, key_in, key_out, idx, tmp, cnt, off, scan_tmp, pk_in, pk_out, pi_in, pi_out, rec, tmp2};
   for (void* f : frees) TR_HIP(hipFree(f));
   return kOk;
 }
@@ -273,9 +326,9 @@ void free_plan(tr_plan* p) {
   for (auto& b : p->bins) {
     (void)hipFree(b.perm);
     (void)hipFree(b.kp);
+    (void)hipFree(b.tap);
     (void)hipFree(b.row_ptr);
-    (void)hipFree(b.col);
-    (void)hipFree(b.val);
+    (void)hipFree(b.pairs);
     (void)hipFree(b.seg_row);
     (void)hipFree(b.seg_j0);
     (void)hipFree(b.heavy_row);
@@ -329,6 +382,10 @@ int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float
     }
   }
   p->G = (int64_t)p->g[0] * p->g[1] * p->g[2];
+  p->ncl[0] = (p->g[0] + kClZ - 1) / kClZ;
+  p->ncl[1] = (p->g[1] + kClY - 1) / kClY;
+  p->ncl[2] = (p->g[2] + kClX - 1) / kClX;
+  p->NCL = (int64_t)p->ncl[0] * p->ncl[1] * p->ncl[2];
   for (int i = 0; i < 3; ++i) {
     const int L = p->g[i];
     std::vector<float2> tw(L);
