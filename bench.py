@@ -176,6 +176,9 @@ def main(argv=None):
     torch.cuda.synchronize(dev)
     samples = [solver.plan.samples(b) for b in range(cfg.bins)]
     log(f"plan + simulated k-space in {time.perf_counter() - t0:.1f} s; samples/bin {samples}", verbose)
+    t0 = time.perf_counter()
+    tuned = solver.autotune()
+    log(f"autotune in {time.perf_counter() - t0:.1f} s: {tuned}", verbose)
 
     for w in range(args.warmup):
         solver.run(progress_every=25)
@@ -246,6 +249,7 @@ def main(argv=None):
                                f"l2 coupling, lambda {cfg.lambda_motion}",
                    "iterations": n_it, "s_per_iteration": t_step / max(n_it, 1),
                    "samples_per_bin": samples, "grid": list(solver.plan.grid_shape),
+                   "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "spread_xcd_chunk")},
                    "final_tol": solver.history[-1] if solver.history else None},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -50,13 +50,24 @@ int tr_profile_read(int id, double* ms_total, int64_t* launches, double* bytes_t
  * Replaces SigPyNufftBackend.__init__ (src/toporecon/nufft_backend.py:58-88):
  * coords[b] is a device (M_b, 3) float32 array in [z, y, x] SigPy units.  The
  * plan scales the coordinates like SigPy's _scale_coord, sorts each bin's
- * samples by grid tile and builds the deterministic spreading CSR. */
+ * samples by grid tile, evaluates every sample's Kaiser-Bessel taps once and
+ * builds the (grid cluster, sample) lists of the deterministic adjoint. */
 int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
                    const int64_t* samples_per_bin, void* stream);
 int tr_plan_destroy(tr_plan* plan);
 /* info[0..2] image dims, [3..5] grid dims, [6] bins, [7] M_b, [8] nnz_b, [9] workspace channels. */
 int tr_plan_info(const tr_plan* plan, int bin, int64_t* info);
 int tr_plan_release_workspace(tr_plan* plan);
+
+/* Launch parameters of a plan (no reference counterpart: SigPy has no tuning
+ * knobs).  They change the block order only, never a result bit.
+ * TR_PARAM_INTERP_XCD_CHUNK / TR_PARAM_SPREAD_XCD_CHUNK: runs of consecutive
+ * workgroups kept on one XCD by the gather kernels, 0 .. 4096 (0 = hardware
+ * round-robin; defaults 64 for the interpolation, 0 for the spreading). */
+#define TR_PARAM_INTERP_XCD_CHUNK 1
+#define TR_PARAM_SPREAD_XCD_CHUNK 2
+int tr_plan_set_param(tr_plan* plan, int param, int64_t value);
+int tr_plan_get_param(const tr_plan* plan, int param, int64_t* value);
 
 /* Sample-order helpers for the fused solver path (no reference counterpart:
  * the reference keeps samples in trajectory order).  gather: (K, M_b) original

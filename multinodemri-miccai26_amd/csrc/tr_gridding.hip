@@ -7,50 +7,72 @@ namespace tr {
 
 // ---------------------------------------------------------------------------
 // Kaiser-Bessel gather at sorted samples.  Lanes own channels (KCI per
-// sample, 64/KCI samples per wave, NCI channel groups per lane).
+// sample, 64/KCI samples per wave, NCI channel groups per lane).  The taps
+// come from the plan's per-sample records; with KCI = 64 the sample index is
+// wave-uniform, so the record and all tap addresses live in scalar registers.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrap2(int p, int g) {
+  // p < p0 + 5 with 0 <= p0 < g and g >= 2: two conditional subtractions suffice
+  p = p >= g ? p - g : p;
+  return p >= g ? p - g : p;
+}
+
 template <int KCI, int NCI>
 __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
   constexpr int SPW = 64 / KCI;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int64_t m = ((int64_t)blockIdx.x * 4 + wave) * SPW + lane / KCI;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
+  const int64_t m = ((int64_t)blk * 4 + wave) * SPW + lane / KCI;
   const int ch = lane % KCI;
   if (m >= a.M) return;
   const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
-  const float4 kp = a.kp[m];
-  float wz[5], wy[5], wx[5];
-  int pz[5], py[5], px[5];
-  const int nz = axis_taps(kp.x, gz, a.beta, 1.0f, wz, pz);
-  const int ny = axis_taps(kp.y, gy, a.beta, 1.0f, wy, py);
-  const int nx = axis_taps(kp.z, gx, a.beta, 1.0f / (kWidth * kWidth * kWidth), wx, px);
+  const TapRec& r = a.tap[m];
+  const int packed = r.n;
+  const int nx = (packed >> 8) & 15, ny = (packed >> 4) & 15;
+  const int nz = (nx == 0 || ny == 0) ? 0 : (packed & 15);  // no taps on one axis: no taps at all
+  const int p0z = r.p0[0], p0y = r.p0[1], p0x = r.p0[2];
   float2 acc[NCI];
 #pragma unroll
   for (int i = 0; i < NCI; ++i) acc[i] = make_float2(0.f, 0.f);
   const int K = a.K;
+  // channels past K read channel K-1 (in bounds, never stored), so that no
+  // load sits behind a per-lane branch and a whole tap row is in flight at once
+  int kk[NCI];
 #pragma unroll
-  for (int dz = 0; dz < 5; ++dz) {
-    if (dz < nz) {
+  for (int i = 0; i < NCI; ++i) kk[i] = min(i * KCI + ch, K - 1);
+  int px[5];
 #pragma unroll
-      for (int dy = 0; dy < 5; ++dy) {
-        if (dy < ny) {
-          const float wzy = wz[dz] * wy[dy];
-          const int64_t rb = ((int64_t)pz[dz] * gy + py[dy]) * gx;
+  for (int dx = 0; dx < 5; ++dx) px[dx] = wrap2(p0x + dx, gx);
+  for (int dz = 0; dz < nz; ++dz) {
+    const int pz = wrap2(p0z + dz, gz);
+    const float wz = r.w[dz];
+    for (int dy = 0; dy < ny; ++dy) {
+      const int py = wrap2(p0y + dy, gy);
+      const float wzy = wz * r.w[5 + dy];
+      const float2* row = a.grid + (((int64_t)pz * gy + py) * gx) * K;
+      // four x taps always exist; a fifth only for a coordinate on a grid line
+      float2 v[4][NCI];
 #pragma unroll
-          for (int dx = 0; dx < 5; ++dx) {
-            if (dx < nx) {
-              const float w = wzy * wx[dx];
-              const float2* gp = a.grid + (rb + px[dx]) * K + ch;
+      for (int dx = 0; dx < 4; ++dx)
 #pragma unroll
-              for (int i = 0; i < NCI; ++i) {
-                if (i * KCI + ch < K) {
-                  const float2 v = gp[i * KCI];
-                  acc[i].x += w * v.x;
-                  acc[i].y += w * v.y;
-                }
-              }
-            }
-          }
+        for (int i = 0; i < NCI; ++i) v[dx][i] = row[(int64_t)px[dx] * K + kk[i]];
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) {
+        const float w = wzy * r.w[10 + dx];
+#pragma unroll
+        for (int i = 0; i < NCI; ++i) {
+          acc[i].x += w * v[dx][i].x;
+          acc[i].y += w * v[dx][i].y;
+        }
+      }
+      if (nx == 5) {
+        const float w = wzy * r.w[14];
+#pragma unroll
+        for (int i = 0; i < NCI; ++i) {
+          const float2 u = row[(int64_t)px[4] * K + kk[i]];
+          acc[i].x += w * u.x;
+          acc[i].y += w * u.y;
         }
       }
     }
@@ -63,9 +85,9 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
       a.out[(int64_t)k * a.M + a.perm[m]] = acc[i];
     } else if (a.mode == kInterpDual) {
       const int64_t idx = m * K + k;
-      const float2 r = csub(acc[i], a.y[idx]);
+      const float2 rr = csub(acc[i], a.y[idx]);
       const float2 qo = a.q[idx];
-      const float2 t = make_float2(qo.x + a.sigma * r.x, qo.y + a.sigma * r.y);
+      const float2 t = make_float2(qo.x + a.sigma * rr.x, qo.y + a.sigma * rr.y);
       a.q[idx] = make_float2(t.x / a.one_plus_sigma, t.y / a.one_plus_sigma);
     } else {
       a.q[m * K + k] = acc[i];
@@ -97,37 +119,45 @@ __device__ __forceinline__ float pick(const float (&w)[N], int i) {
   return r;
 }
 
-// source value of sorted sample m, channel k (zero past K)
-template <int SORTED>
-__device__ __forceinline__ float2 spread_src(const SpreadArgs& a, int m, int k) {
-  if (k >= a.K) return make_float2(0.f, 0.f);
+// source value of sorted sample m, channel kk (kk is clamped to K-1 by the
+// caller: lanes past K load a valid channel whose sums are never stored)
+template <int SORTED, int DCF>
+__device__ __forceinline__ float2 spread_src(const SpreadArgs& a, int m, int kk) {
   if (SORTED) {
-    const float d = a.dcf_sorted ? a.dcf_sorted[m] : 1.0f;
-    return cscale(a.q[(int64_t)m * a.K + k], d);
+    const float2 v = a.q[(int64_t)m * a.K + kk];
+    return DCF ? cscale(v, a.dcf_sorted[m]) : v;
   }
   const int mo = a.perm[m];
-  const float d = a.dcf_orig ? a.dcf_orig[mo] : 1.0f;
-  return cscale(a.ksp[(int64_t)k * a.M + mo], d);
+  const float2 v = a.ksp[(int64_t)kk * a.M + mo];
+  return DCF ? cscale(v, a.dcf_orig[mo]) : v;
 }
 
+// acc[z][y][x] += (wz[z] wy[y]) * (wx[x] v): the point weights factor over
+// the axes, so each pair costs 2 multiply-adds per point.
 template <int KCI>
 __device__ __forceinline__ void cl_add(const ClusterPair& r, float2 v, int pl, float2 (&acc)[ClLanes<KCI>::NP]) {
   constexpr int NPL = ClLanes<KCI>::NPL;
+  float2 tx[kClX];
+#pragma unroll
+  for (int ix = 0; ix < kClX; ++ix) tx[ix] = make_float2(r.wx[ix] * v.x, r.wx[ix] * v.y);
 #pragma unroll
   for (int i = 0; i < ClLanes<KCI>::NP; ++i) {
     const int pt = pl + i * NPL;
     const int iz = pt / (kClY * kClX), iy = (pt / kClX) % kClY, ix = pt % kClX;
-    const float w = (pick(r.wz, iz) * pick(r.wy, iy)) * pick(r.wx, ix);
-    acc[i].x += w * v.x;
-    acc[i].y += w * v.y;
+    const float wzy = pick(r.wz, iz) * pick(r.wy, iy);
+    float2 t = tx[0];
+#pragma unroll
+    for (int c = 1; c < kClX; ++c) t = ix == c ? tx[c] : t;
+    acc[i].x += wzy * t.x;
+    acc[i].y += wzy * t.y;
   }
 }
 
 // acc += sum over pairs [j0, j1) in increasing order (four pairs' loads are
 // issued together; the additions stay in order)
-template <int KCI, int SORTED>
-__device__ __forceinline__ void cl_run(const SpreadArgs& a, int j0, int j1, int k, int pl,
-                                       float2 (&acc)[ClLanes<KCI>::NP]) {
+template <int KCI, int SORTED, int DCF>
+__device__ __forceinline__ void cl_run_impl(const SpreadArgs& a, int j0, int j1, int kk, int pl,
+                                            float2 (&acc)[ClLanes<KCI>::NP]) {
   int j = j0;
   for (; j + 4 <= j1; j += 4) {
     ClusterPair r[4];
@@ -135,14 +165,24 @@ __device__ __forceinline__ void cl_run(const SpreadArgs& a, int j0, int j1, int 
 #pragma unroll
     for (int u = 0; u < 4; ++u) r[u] = a.pairs[j + u];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = spread_src<SORTED>(a, r[u].m, k);
+    for (int u = 0; u < 4; ++u) v[u] = spread_src<SORTED, DCF>(a, r[u].m, kk);
 #pragma unroll
     for (int u = 0; u < 4; ++u) cl_add<KCI>(r[u], v[u], pl, acc);
   }
   for (; j < j1; ++j) {
     const ClusterPair r = a.pairs[j];
-    cl_add<KCI>(r, spread_src<SORTED>(a, r.m, k), pl, acc);
+    cl_add<KCI>(r, spread_src<SORTED, DCF>(a, r.m, kk), pl, acc);
   }
+}
+
+template <int KCI, int SORTED>
+__device__ __forceinline__ void cl_run(const SpreadArgs& a, int j0, int j1, int k, int pl,
+                                       float2 (&acc)[ClLanes<KCI>::NP]) {
+  const int kk = min(k, a.K - 1);
+  if (SORTED ? a.dcf_sorted != nullptr : a.dcf_orig != nullptr)
+    cl_run_impl<KCI, SORTED, 1>(a, j0, j1, kk, pl, acc);
+  else
+    cl_run_impl<KCI, SORTED, 0>(a, j0, j1, kk, pl, acc);
 }
 
 // Write a cluster's sums to the grid (points past the grid edge are dropped).
@@ -172,7 +212,8 @@ template <int KCI, int SORTED>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs a) {
   constexpr int NP = ClLanes<KCI>::NP;
   const int lane = threadIdx.x & 63;
-  const int64_t item = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
+  const int64_t item = (int64_t)blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (item >= a.NCL * a.ngrp) return;
   const int64_t cl = item / a.ngrp;
   const int grp = (int)(item - cl * a.ngrp);
@@ -280,6 +321,7 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
   ProfScope ps(kKSpread, 8.0 * K * a.M + 16.0 * a.M + 8.0 * K * G, s);
   const int kci = K >= 64 ? 64 : pow2ceil(K);
   a.gg = geom(p);
+  a.xcd_chunk = p->spread_xcd_chunk;
   for (int i = 0; i < 3; ++i) a.ncl[i] = p->ncl[i];
   a.NCL = p->NCL;
   a.row_ptr = b.row_ptr;

@@ -87,6 +87,10 @@ struct tr_plan {
   int64_t partial_n = 0;
   int ncl[3];                 // clusters per axis
   int64_t NCL;
+  // launch parameters (tr_plan_set_param): runs of consecutive blocks kept on
+  // one XCD by the gather kernels (0 = hardware round-robin order)
+  int interp_xcd_chunk = 64;
+  int spread_xcd_chunk = 0;
 };
 
 namespace tr {
@@ -150,8 +154,7 @@ enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2 };
 
 struct InterpArgs {
   GridGeom gg;
-  float beta;
-  const float4* kp;
+  int xcd_chunk;
   const TapRec* tap;
   int64_t M;
   int K;
@@ -169,6 +172,7 @@ struct InterpArgs {
 
 struct SpreadArgs {
   GridGeom gg;
+  int xcd_chunk;
   int ncl[3];
   int64_t NCL;
   int K;
@@ -209,6 +213,19 @@ struct Tile {
 // Host helpers
 // ---------------------------------------------------------------------------
 inline unsigned grid1d(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// With chunk S > 0 the logical block order is permuted so that every XCD
+// receives runs of S consecutive logical blocks, which then share its L2; the
+// tail that does not fill whole 8 * S groups keeps the identity order.
+constexpr unsigned kNumXcd = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned S) {
+  if (S == 0) return b;
+  const unsigned full = nb / (kNumXcd * S) * (kNumXcd * S);
+  if (b >= full) return b;
+  const unsigned r = b / kNumXcd, x = b % kNumXcd;
+  return (r / S) * (kNumXcd * S) + x * S + (r % S);
+}
 
 inline int pow2ceil(int v) {
   int p = 1;

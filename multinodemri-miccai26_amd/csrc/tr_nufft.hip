@@ -164,8 +164,8 @@ int tr_nufft_forward(tr_plan* p, int bin, const tr_c64* img, const tr_c64* maps,
   if (int r = forward_to_grid(p, F2(img), F2(maps), E, C, s)) return r;
   InterpArgs a = {};
   a.gg = geom(p);
-  a.beta = p->beta;
-  a.kp = b.kp;
+  a.xcd_chunk = p->interp_xcd_chunk;
+  a.tap = b.tap;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -225,8 +225,8 @@ int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* m
   if (int r = forward_to_grid(p, F2(ubar), F2(maps), E, C, s)) return r;
   InterpArgs a = {};
   a.gg = geom(p);
-  a.beta = p->beta;
-  a.kp = b.kp;
+  a.xcd_chunk = p->interp_xcd_chunk;
+  a.tap = b.tap;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -247,8 +247,8 @@ int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   if (int r = forward_to_grid(p, F2(x), F2(maps), E, C, s)) return r;
   InterpArgs a = {};
   a.gg = geom(p);
-  a.beta = p->beta;
-  a.kp = b.kp;
+  a.xcd_chunk = p->interp_xcd_chunk;
+  a.tap = b.tap;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;

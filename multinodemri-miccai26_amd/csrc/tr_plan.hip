@@ -461,6 +461,25 @@ int tr_plan_release_workspace(tr_plan* p) {
   return kOk;
 }
 
+int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
+  if (!p) TR_FAIL(kInvalidArgument, "null plan");
+  if (value < 0 || value > 4096) TR_FAIL(kInvalidArgument, "parameter value outside [0, 4096]");
+  switch (param) {
+    case TR_PARAM_INTERP_XCD_CHUNK: p->interp_xcd_chunk = (int)value; return kOk;
+    case TR_PARAM_SPREAD_XCD_CHUNK: p->spread_xcd_chunk = (int)value; return kOk;
+    default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));
+  }
+}
+
+int tr_plan_get_param(const tr_plan* p, int param, int64_t* value) {
+  if (!p || !value) TR_FAIL(kInvalidArgument, "null argument");
+  switch (param) {
+    case TR_PARAM_INTERP_XCD_CHUNK: *value = p->interp_xcd_chunk; return kOk;
+    case TR_PARAM_SPREAD_XCD_CHUNK: *value = p->spread_xcd_chunk; return kOk;
+    default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));
+  }
+}
+
 int tr_plan_gather_samples(const tr_plan* p, int bin, const tr_c64* src, int K, tr_c64* dst, void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;
   const BinData& b = p->bins[bin];

@@ -33,6 +33,8 @@ SIGNATURES = {
     "tr_plan_destroy": (c_int, [c_vp]),
     "tr_plan_info": (c_int, [c_vp, c_int, ctypes.POINTER(c_i64)]),
     "tr_plan_release_workspace": (c_int, [c_vp]),
+    "tr_plan_set_param": (c_int, [c_vp, c_int, c_i64]),
+    "tr_plan_get_param": (c_int, [c_vp, c_int, ctypes.POINTER(c_i64)]),
     "tr_plan_gather_samples": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp]),
     "tr_plan_scatter_samples": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp]),
     "tr_plan_gather_weights": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),

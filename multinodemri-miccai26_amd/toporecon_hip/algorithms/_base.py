@@ -208,6 +208,27 @@ class HipPdhgBase:
                 self.y[b].div_(np.float32(mx))
         return mx
 
+    def autotune(self, reps: int = 2, cache_path=None, force: bool = False) -> dict:
+        """Pick the plan's gridding launch parameters by timing them on the largest bin.
+
+        Only the workgroup order changes, so run() gives bit-identical results
+        whatever is chosen.
+        """
+        import torch
+        from .. import autotune as _autotune
+        if self.B == 0:
+            return {}
+        b = max(range(self.B), key=self.plan.samples)
+        if self.plan.samples(b) == 0:
+            return {}
+        img = torch.zeros((self.E, *self.img_shape), dtype=torch.complex64, device=self.device)
+        ksp = torch.zeros_like(self.y[b])
+        out = _autotune.tune(self.plan, image=img, maps=self.maps, E=self.E, C=self.C, ksp=ksp,
+                             dcf=self.dcf[b], stream=self.stream, bins=(b,), reps=reps,
+                             cache_path=cache_path, force=force)
+        del img, ksp
+        return out
+
     # -- run loop ----------------------------------------------------------------
     def _converged_value(self, num, den):
         if math.isfinite(num) and math.isfinite(den) and den > 0.0:

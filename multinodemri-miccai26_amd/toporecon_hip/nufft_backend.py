@@ -119,6 +119,17 @@ class NufftPlan:
     def release_workspace(self):
         _lib.call("tr_plan_release_workspace", self.handle)
 
+    PARAMS = {"interp_xcd_chunk": 1, "spread_xcd_chunk": 2}
+
+    def set_param(self, name: str, value: int):
+        """Set a launch parameter (block order only; results are unchanged)."""
+        _lib.call("tr_plan_set_param", self.handle, self.PARAMS[name], int(value))
+
+    def get_param(self, name: str) -> int:
+        out = ctypes.c_int64()
+        _lib.call("tr_plan_get_param", self.handle, self.PARAMS[name], ctypes.byref(out))
+        return int(out.value)
+
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().tr_plan_destroy(self.handle)

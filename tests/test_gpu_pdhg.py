@@ -81,3 +81,24 @@ def test_rerun_is_bitwise_deterministic(torch_mod):
     a = _hip_solver("tvme", acq, ksp, 3, 0.0, True).run().cpu().numpy()
     b = _hip_solver("tvme", acq, ksp, 3, 0.0, True).run().cpu().numpy()
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_launch_params_and_autotune_keep_results_bitwise(torch_mod):
+    """The XCD block order of the gather kernels is a tuning knob only: every
+    choice, and whatever the autotuner picks, gives the same bits."""
+    acq, ksp, prob = make_problem(seed=11)
+    base = _hip_solver("tvme", acq, ksp, 3, 0.0, True)
+    ref = base.run().cpu().numpy()
+    for ci, cs in [(0, 0), (1, 1), (16, 64), (1024, 4096)]:
+        base.plan.set_param("interp_xcd_chunk", ci)
+        base.plan.set_param("spread_xcd_chunk", cs)
+        assert base.plan.get_param("interp_xcd_chunk") == ci
+        out = base.run().cpu().numpy()
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (ci, cs)
+    tuned = base.autotune(reps=1, force=True)
+    assert tuned["interp_xcd_chunk"] in (0, 16, 64, 256, 1024)
+    out = base.run().cpu().numpy()
+    assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
+    with pytest.raises(Exception):
+        base.plan.set_param("interp_xcd_chunk", 5000)
