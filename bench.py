@@ -47,6 +47,12 @@ def parse_args(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarize two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, "
+                         "WRITE_SIZE) of a run with --copy-calibration-mib into profiles/traffic.json")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the solve, copy a buffer of this many MiB once (a kernel of known "
+                         "traffic for scaling hardware byte counters)")
     return ap.parse_args(argv)
 
 
@@ -138,8 +144,81 @@ def run_cpu_baseline(cfg_name, iterations, verbose):
             "s_per_iteration": d["per_iter"]}
 
 
+# Kernel symbols -> the logical kernel names used in the JSON line.  The line
+# passes share one symbol per length; forward and inverse passes are told
+# apart by the kernel that ran before them on the stream.
+def _logical_kernel(sym, prev):
+    if "k_fwd_x_pad" in sym:
+        return "fwd_x_pad"
+    if "k_adj_x_crop" in sym:
+        return "adj_x_crop"
+    if "k_interp" in sym:
+        return "interp"
+    if "k_spread" in sym:
+        return "spread"
+    if "k_fft_lines" in sym:
+        return {"fwd_x_pad": "fft_y_fwd", "fft_y_fwd": "fft_z_fwd", "spread": "fft_z_inv",
+                "fft_z_inv": "fft_y_inv"}.get(prev)
+    for name in ("tvme_dual", "tvme_primal", "tvm_dual", "tvm_primal", "tvmw_dual", "tvmw_primal"):
+        if f"k_{name}(" in sym:
+            return name
+    return None
+
+
+def _launch_counters(path, counter):
+    """{logical kernel: [bytes per launch as counted]} and the calibration copy's count."""
+    import csv
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+    rows = [r for r in rows if r["Counter_Name"] == counter]
+    copy_counted = [float(r["Counter_Value"]) * 1024.0 for r in rows
+                    if "copyBuffer" in r["Kernel_Name"] and float(r["Counter_Value"]) > 1024 * 1024]
+    launches, prev = {}, None
+    for r in rows:
+        name = _logical_kernel(r["Kernel_Name"], prev)
+        if name is None:
+            continue
+        v = float(r["Counter_Value"]) * 1024.0  # the counters are in KiB
+        if name == "spread" and prev == "spread":
+            launches[name][-1] += v  # segment and combine kernels of the same spread launch
+            continue
+        launches.setdefault(name, []).append(v)
+        prev = name
+    return launches, copy_counted
+
+
+def write_traffic(fetch_csv, write_csv, copy_mib, out_path):
+    """HBM bytes per launch of every logical kernel from FETCH_SIZE / WRITE_SIZE, each
+    scaled by copy bytes / what the counter reads for the calibration copy."""
+    import statistics
+    fetch, fcopy = _launch_counters(fetch_csv, "FETCH_SIZE")
+    write, wcopy = _launch_counters(write_csv, "WRITE_SIZE")
+    if not fcopy or not wcopy:
+        raise SystemExit("no calibration copy found: profile the bench with --copy-calibration-mib")
+    copy_bytes = copy_mib * float(1 << 20)
+    f_factor = copy_bytes / max(fcopy)
+    w_factor = copy_bytes / max(wcopy)
+    kernels = {}
+    for name in sorted(set(fetch) | set(write)):
+        fb = statistics.median(fetch.get(name, [0.0]))
+        wb = statistics.median(write.get(name, [0.0]))
+        kernels[name] = {"fetch_bytes_counted": fb, "write_bytes_counted": wb,
+                         "bytes": fb * f_factor + wb * w_factor,
+                         "launches_sampled": len(fetch.get(name, []))}
+    out = {"calibration": {"copy_bytes": copy_bytes, "fetch_factor": f_factor, "write_factor": w_factor,
+                           "note": "factor = copy_bytes / counter value of one device copy of copy_bytes"},
+           "kernels": kernels}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    return out
+
+
 def main(argv=None):
     args = parse_args(argv)
+    if args.traffic_from:
+        out = write_traffic(args.traffic_from[0], args.traffic_from[1], args.copy_calibration_mib or 2048,
+                            TRAFFIC_FILE)
+        print(json.dumps(out["calibration"]))
+        return 0
     if args.cpu_baseline_worker:
         cpu_baseline_worker(args.config)
         return 0
@@ -176,6 +255,15 @@ def main(argv=None):
     torch.cuda.synchronize(dev)
     samples = [solver.plan.samples(b) for b in range(cfg.bins)]
     log(f"plan + simulated k-space in {time.perf_counter() - t0:.1f} s; samples/bin {samples}", verbose)
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize(dev)
+        dst.copy_(src)
+        torch.cuda.synchronize(dev)
+        log(f"copy calibration: one copy of {4 * n} bytes (read {4 * n}, write {4 * n})", verbose)
+        del src, dst
     t0 = time.perf_counter()
     tuned = solver.autotune()
     log(f"autotune in {time.perf_counter() - t0:.1f} s: {tuned}", verbose)
@@ -209,7 +297,7 @@ def main(argv=None):
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get(dom)
+            traffic = json.load(f).get("kernels", {}).get(dom, {}).get("bytes")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["gbps"], "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": dk["gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
                 "algorithmic_bytes_per_launch": dk["bytes_per_launch"], "avg_launch_ms": dk["ms_avg"]}

@@ -255,31 +255,39 @@ __global__ void __launch_bounds__(256) k_spread_segments(SpreadArgs a) {
   }
 }
 
-// Heavy cluster = sum of its segment partials in segment order.
-template <int KCI, int SORTED>
+// Heavy cluster = sum of its segment partials in segment order; one thread
+// per (heavy cluster, point, channel), channel fastest.
 __global__ void __launch_bounds__(256) k_spread_combine(SpreadArgs a) {
-  constexpr int NP = ClLanes<KCI>::NP;
-  constexpr int NPL = ClLanes<KCI>::NPL;
-  const int lane = threadIdx.x & 63;
-  const int64_t item = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (item >= a.nheavy * a.ngrp) return;
-  const int64_t h = item / a.ngrp;
-  const int grp = (int)(item - h * a.ngrp);
-  const int cl = a.heavy_row[h];
-  const int k = grp * KCI + lane % KCI;
-  const int pl = lane / KCI;
-  if (k >= a.K) return;
-  float2 acc[NP];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int K = a.K;
+  if (i >= a.nheavy * kClPts * K) return;
+  const int k = (int)(i % K);
+  const int64_t hp = i / K;
+  const int pt = (int)(hp % kClPts);
+  const int64_t h = hp / kClPts;
+  const int s0 = a.heavy_seg_ptr[h], s1 = a.heavy_seg_ptr[h + 1];
+  const float2* src = a.partial + (int64_t)pt * K + k;
+  const int64_t stride = (int64_t)kClPts * K;
+  float2 acc = make_float2(0.f, 0.f);
+  int sg = s0;
+  for (; sg + 4 <= s1; sg += 4) {
+    float2 v[4];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) acc[i] = make_float2(0.f, 0.f);
-  for (int sg = a.heavy_seg_ptr[h]; sg < a.heavy_seg_ptr[h + 1]; ++sg) {
+    for (int u = 0; u < 4; ++u) v[u] = src[(int64_t)(sg + u) * stride];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const int pt = pl + i * NPL;
-      if (pt < kClPts) acc[i] = cadd(acc[i], a.partial[((int64_t)sg * kClPts + pt) * a.K + k]);
-    }
+    for (int u = 0; u < 4; ++u) acc = cadd(acc, v[u]);
   }
-  cl_store<KCI>(a, cl, k, pl, acc);
+  for (; sg < s1; ++sg) acc = cadd(acc, src[(int64_t)sg * stride]);
+  const int64_t cl = a.heavy_row[h];
+  const int ncy = a.ncl[1], ncx = a.ncl[2];
+  const int cz = (int)(cl / ((int64_t)ncy * ncx));
+  const int rem = (int)(cl - (int64_t)cz * ncy * ncx);
+  const int cy = rem / ncx, cx = rem - (rem / ncx) * ncx;
+  const int z = cz * kClZ + pt / (kClY * kClX);
+  const int y = cy * kClY + (pt / kClX) % kClY;
+  const int x = cx * kClX + pt % kClX;
+  const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
+  if (z < gz && y < gy && x < gx) a.grid[(((int64_t)z * gy + y) * gx + x) * K + k] = acc;
 }
 
 int launch_interp(const InterpArgs& a, hipStream_t s) {
@@ -359,7 +367,8 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
   TR_SPREAD_ALL(k_spread, a.NCL);
   if (b.nseg > 0) {
     TR_SPREAD_ALL(k_spread_segments, b.nseg);
-    TR_SPREAD_ALL(k_spread_combine, b.nheavy);
+    hipLaunchKernelGGL(k_spread_combine, dim3(grid1d(b.nheavy * kClPts * K, 256)), dim3(256), 0, s, a);
+    TR_LAUNCH_CHECK();
   }
 #undef TR_SPREAD_ALL
 #undef TR_SPREAD_KCI
