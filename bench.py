@@ -1,18 +1,26 @@
 #!/usr/bin/env python3
 """TVME time-to-tolerance benchmark of the HIP PDHG hot path on one MI355X.
 
-Workload (default): plain TVME PDHG from zero to tol 1e-3 (max 300 iterations,
-sigma = tau = 1/6, lambda_motion = lambda_echo = 1e-5, L2 echo coupling) at
-80x192x192, 32 coils, 6 echoes (TE = 1.0 + 1.3 e ms), 6 motion bins, 9000 cone
+Workload (default): TVME PDHG to tol 1e-3 (max 300 iterations, sigma = tau =
+1/6, lambda_motion = lambda_echo = 1e-5, L2 echo coupling) at 80x192x192,
+32 coils, 6 echoes (TE = 1.0 + 1.3 e ms), 6 motion bins, 9000 cone
 interleaves x 512 samples per echo (readout fraction 0.98, 2/98 % respiratory
-outlier drop), synthetic seeded data simulated on the GPU.  A step is one full
-solve; inputs are resident in HBM when the timed region starts.
+outlier drop), synthetic seeded data simulated on the GPU.  A step is one
+full solve; inputs are resident in HBM when the timed region starts.
+
+Two legs run on the same problem: plain PDHG from zeros (as the reference),
+and the learned warm start (density-compensated adjoint -> 3-D CNN on MFMA ->
+per-(bin, echo) data-fitted scale -> PDHG from there to the same tolerance,
+the warm-start work inside the timed region).  ``value`` is the learned leg's
+time-to-tol when its final objective is within 0.1 % of the plain leg's,
+otherwise the plain leg's (the fallback); both legs are reported.
 
 Prints one JSON line: time-to-tol (value, seconds), iterations, per-kernel
-algorithmic GB/s from HIP events recorded on the launch stream, the dominant
-kernel's fraction of the 8 TB/s HBM peak, forward/adjoint NUFFT GB/s, and a
-single-core CPU baseline (the float64 oracle timed on a bounded sample and
-extrapolated to the same iteration count).
+algorithmic GB/s from HIP events recorded on the launch stream (plain leg),
+the dominant kernel's fraction of the 8 TB/s HBM peak with its measured HBM
+traffic, forward/adjoint NUFFT GB/s, and a single-core CPU baseline (the
+float64 oracle timed on a bounded sample and extrapolated to the plain
+iteration count).
 """
 
 from __future__ import annotations
@@ -268,24 +276,69 @@ def main(argv=None):
     tuned = solver.autotune()
     log(f"autotune in {time.perf_counter() - t0:.1f} s: {tuned}", verbose)
 
-    for w in range(args.warmup):
-        solver.run(progress_every=25)
-        log(f"warmup {w}: {solver.iterations} iterations in {solver.elapsed:.2f} s", verbose)
+    from toporecon_hip import objective
+    from toporecon_hip.learned import WarmStartNet, cnn_warm_start, default_weights_path
+    net = None
+    if os.path.exists(default_weights_path()):
+        net = WarmStartNet.load(default_weights_path(), dev)
+    else:
+        log("no trained warm-start weights: the learned leg is skipped (plain PDHG only)", verbose)
 
+    # warmup: short solves through both paths (kernels, plans and caches warm)
+    full_iter = solver.max_iter
+    solver.max_iter = min(full_iter, 5)
+    for w in range(args.warmup):
+        solver.run()
+        if net is not None:
+            solver.run(init=cnn_warm_start(solver, net), dual_init="residual")
+        torch.cuda.synchronize(dev)
+        log(f"warmup {w} done", verbose)
+    solver.max_iter = full_iter
+
+    # plain PDHG from zeros (kernel profile from this leg)
     _lib.profile_enable(True)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     iters = []
     for k in range(args.steps):
-        solver.run(progress_every=25)
+        u_plain = solver.run(progress_every=25)
         iters.append(solver.iterations)
     torch.cuda.synchronize(dev)
     elapsed = time.perf_counter() - t0
     prof = _lib.profile_read()
     _lib.profile_enable(False)
-    t_step = elapsed / args.steps
+    t_plain = elapsed / args.steps
     n_it = iters[-1]
-    log(f"timed: {args.steps} solve(s), {n_it} iterations, {t_step:.2f} s per solve", verbose)
+    u_plain = u_plain.clone()
+    phi_plain = objective.objective(solver, u_plain)
+    del u_plain
+    log(f"plain: {args.steps} solve(s), {n_it} iterations, {t_plain:.2f} s per solve, objective {phi_plain:.6e}",
+        verbose)
+
+    # learned warm start: dcf adjoint -> CNN (MFMA, bf16) -> data-fitted scale -> PDHG to the same tol
+    warm = None
+    if net is not None:
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ta = time.perf_counter()
+            u0 = cnn_warm_start(solver, net)
+            torch.cuda.synchronize(dev)
+            t_start = time.perf_counter() - ta
+            u_warm = solver.run(init=u0, dual_init="residual", progress_every=25)
+        torch.cuda.synchronize(dev)
+        t_warm = (time.perf_counter() - t0) / args.steps
+        phi_warm = objective.objective(solver, u_warm)
+        rel = (phi_warm - phi_plain) / phi_plain
+        warm = {"time_to_tol_s": t_warm, "iterations": solver.iterations, "warm_start_s": t_start,
+                "objective": phi_warm, "objective_plain": phi_plain, "objective_rel_diff": rel,
+                "within_0.1pct": abs(rel) <= 1e-3, "speedup_vs_plain": t_plain / t_warm,
+                "weights": os.path.basename(default_weights_path()), "trained_on": net.meta}
+        del u0, u_warm
+        log(f"warm start: {solver.iterations} iterations, {t_warm:.2f} s (start {t_start:.2f} s), objective "
+            f"{phi_warm:.6e} ({rel:+.2e} vs plain), speedup {t_plain / t_warm:.2f}x", verbose)
+    use_warm = warm is not None and warm["within_0.1pct"]
+    t_step = warm["time_to_tol_s"] if use_warm else t_plain
 
     kernels = {}
     for name, (ms, n, by) in prof.items():
@@ -331,11 +384,17 @@ def main(argv=None):
         "dtype": "f32",
         "data": "synthetic: seeded cone trajectory, ellipsoid phantom, Gaussian-blob coil maps, "
                 "k-space simulated by the HIP forward NUFFT + 30 dB complex noise",
-        "config": {"workload": f"TVME plain PDHG (no warm start) to tol {cfg.tol} (max {cfg.max_iter} it), "
+        "config": {"workload": f"TVME PDHG to tol {cfg.tol} (max {cfg.max_iter} it) "
+                               f"{'from the learned CNN warm start' if use_warm else 'from zeros (plain)'}, "
                                f"{shape}, {cfg.coils} coils, {cfg.echoes} echoes, {cfg.bins} bins, "
                                f"{cfg.interleaves}x{cfg.readouts} cone samples/echo, "
                                f"l2 coupling, lambda {cfg.lambda_motion}",
-                   "iterations": n_it, "s_per_iteration": t_step / max(n_it, 1),
+                   "start": "learned" if use_warm else "plain",
+                   "plain": {"time_to_tol_s": t_plain, "iterations": n_it, "objective": phi_plain,
+                             "s_per_iteration": t_plain / max(n_it, 1)},
+                   "learned_warm_start": warm,
+                   "iterations": warm["iterations"] if use_warm else n_it,
+                   "s_per_iteration": t_plain / max(n_it, 1),
                    "samples_per_bin": samples, "grid": list(solver.plan.grid_shape),
                    "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "spread_xcd_chunk")},
                    "final_tol": solver.history[-1] if solver.history else None},

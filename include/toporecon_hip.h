@@ -109,6 +109,10 @@ int tr_data_dual_update(tr_plan* plan, int bin, const tr_c64* ubar_b, const tr_c
 /* y_sorted = A x_b (synthetic k-space; no reference counterpart). */
 int tr_data_simulate(tr_plan* plan, int bin, const tr_c64* x_b, const tr_c64* maps, int E, int C,
                      tr_c64* y_sorted, void* stream);
+/* q_b = A x_b - y_b on sorted samples: the data dual's PDHG fixed point for an initial
+ * image x (no reference counterpart: the reference starts q at zero, tvme.py:180-186). */
+int tr_data_residual(tr_plan* plan, int bin, const tr_c64* x, const tr_c64* maps, int E, int C,
+                     const tr_c64* y_sorted, tr_c64* q_sorted, void* stream);
 /* g_b (+)= A^H W q: sum_c conj(S_c) NUFFT^H(dcf * q_c) on sorted samples.
  * Replaces tvm.py:216-225 (+ coil all-reduce tvm.py:188-214), tvme.py:383-419, tvmw.py:586-629. */
 int tr_data_adjoint(tr_plan* plan, int bin, const tr_c64* q_sorted, const float* dcf_sorted,
@@ -157,6 +161,40 @@ int tr_w2_adjoint(const tr_c64* coeffs, int64_t nimg, int Z, int Y, int X, int l
 /* p <- p + sigma w projected onto |p| <= lam per coefficient with sqrt(re^2 + im^2 + 1e-12)
  * (0 when lam <= 0) (tvmw.py:162-193). */
 int tr_w2_dual_update(tr_c64* p, const tr_c64* w, int64_t n, float sigma, float lam, void* stream);
+
+/* ---- learned warm start (no counterpart in the reference, whose solvers start from
+ * zeros: tvme.py:180-186).  A 3-D CNN of 3x3x3 'same' convolutions maps a (bin, echo)
+ * image to an estimate of the converged one.  Activations are channels-last bf16
+ * volumes [n][Z][Y][X][C] (C = 8 for the input: re, im, 6 zeros; 32 otherwise);
+ * master weights fp32 [27][Cin][32] with tap t = 9(dz+1) + 3(dy+1) + (dx+1).
+ * bf16 values are passed as uint16_t bit patterns. ---- */
+/* wfrag: tr_cnn_pack_weights output.  mode 0: + bias, ReLU -> bf16 [..][32]; mode 1: + bias
+ * -> fp32 [..][cout_keep]; mode 2 (data gradient with the flipped kernel): zero where
+ * mask <= 0 -> bf16 [..][32]. */
+int tr_cnn_conv(const uint16_t* x, int n, int Z, int Y, int X, int cin, const uint16_t* wfrag,
+                const float* bias, int mode, const uint16_t* mask, void* out, int cout_keep, void* stream);
+/* MFMA fragments of w (27 * cin * 32 bf16); flip = 1 packs W'[t][co][ci] = W[26-t][ci][co]. */
+int tr_cnn_pack_weights(const float* w, int cin, int flip, uint16_t* frag, void* stream);
+/* dw[27][cin][32] = sum_v x[v + off(t)][ci] dy[v][co], db[32] = sum_v dy[v][co]; nblocks
+ * workgroups write partial sums to work (tr_cnn_wgrad_work_size floats), added in block order. */
+int64_t tr_cnn_wgrad_work_size(int nblocks);
+int tr_cnn_wgrad(const uint16_t* x, const uint16_t* dy, int n, int Z, int Y, int X, int cin, int nblocks,
+                 float* work, float* dw, float* db, void* stream);
+/* dy[v] = (scale (out - target)[v][0:2], 0 x 30) bf16; partial[2 * b] = block b's sum of squares. */
+int tr_cnn_mse(const float* out, const float* target, int64_t V, float scale, uint16_t* dy, double* partial,
+               int nblocks, void* stream);
+int tr_cnn_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                float eps, int step, void* stream);
+/* x[i][v] = bf16(img[i][v] / scale[i]) as (re, im, 0 x 6); img = scale (x[0:2] + out[0:2]). */
+int tr_cnn_pack_input(const tr_c64* img, const float* scale, int64_t V, int n, uint16_t* x, void* stream);
+int tr_cnn_unpack(const uint16_t* x, const float* out, const float* scale, int64_t V, int n, tr_c64* img,
+                  void* stream);
+/* out[i] = sum_v |img[i][v]|^2 (float64, fixed order). */
+int tr_cnn_sumsq(const tr_c64* img, int64_t V, int n, double* out, void* stream);
+/* out[k] = (sum_m dcf_m Re(conj(a_mk) y_mk), sum_m dcf_m |a_mk|^2) over sorted samples (float64;
+ * work holds 2 * nblocks * K doubles): the least-squares scale of a predicted image. */
+int tr_data_fit_sums(const tr_c64* a_sorted, const tr_c64* y_sorted, const float* dcf_sorted, int64_t M, int K,
+                     double* work, int nblocks, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,9 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
       const float2 qo = a.q[idx];
       const float2 t = make_float2(qo.x + a.sigma * rr.x, qo.y + a.sigma * rr.y);
       a.q[idx] = make_float2(t.x / a.one_plus_sigma, t.y / a.one_plus_sigma);
+    } else if (a.mode == kInterpResid) {
+      const int64_t idx = m * K + k;
+      a.q[idx] = csub(acc[i], a.y[idx]);
     } else {
       a.q[m * K + k] = acc[i];
     }
@@ -293,7 +296,9 @@ __global__ void __launch_bounds__(256) k_spread_combine(SpreadArgs a) {
 int launch_interp(const InterpArgs& a, hipStream_t s) {
   const int K = a.K;
   const double G = (double)a.gg.g[0] * a.gg.g[1] * a.gg.g[2];
-  const double per_sample = a.mode == kInterpDual ? 24.0 * K : (a.mode == kInterpPlain ? 8.0 * K + 4.0 : 8.0 * K);
+  const double per_sample = a.mode == kInterpDual ? 24.0 * K
+                            : a.mode == kInterpResid ? 16.0 * K
+                            : (a.mode == kInterpPlain ? 8.0 * K + 4.0 : 8.0 * K);
   ProfScope ps(kKInterp, 8.0 * K * G + a.M * (12.0 + per_sample), s);
   const int kci = K >= 64 ? 64 : pow2ceil(K);
   const int nci = (K + kci - 1) / kci;

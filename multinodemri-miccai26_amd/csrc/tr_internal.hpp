@@ -150,7 +150,7 @@ struct AdjXArgs {
   int accumulate;
 };
 
-enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2 };
+enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2, kInterpResid = 3 };
 
 struct InterpArgs {
   GridGeom gg;

@@ -257,6 +257,26 @@ int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   return launch_interp(a, s);
 }
 
+int tr_data_residual(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, int E, int C,
+                     const tr_c64* y_sorted, tr_c64* q_sorted, void* stream) {
+  if (int r = check_plan_bin(p, bin)) return r;
+  hipStream_t s = (hipStream_t)stream;
+  const BinData& b = p->bins[bin];
+  if (b.M == 0) return kOk;
+  if (int r = forward_to_grid(p, F2(x), F2(maps), E, C, s)) return r;
+  InterpArgs a = {};
+  a.gg = geom(p);
+  a.xcd_chunk = p->interp_xcd_chunk;
+  a.tap = b.tap;
+  a.M = b.M;
+  a.K = E * C;
+  a.grid = p->work;
+  a.mode = kInterpResid;
+  a.y = F2(y_sorted);
+  a.q = F2(q_sorted);
+  return launch_interp(a, s);
+}
+
 int tr_data_adjoint(tr_plan* p, int bin, const tr_c64* q_sorted, const float* dcf_sorted,
                     const tr_c64* maps, int E, int C, tr_c64* g, int accumulate, void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;

@@ -45,6 +45,7 @@ SIGNATURES = {
     "tr_data_dual_update": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_f32, c_f32,
                                     c_vp]),
     "tr_data_simulate": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
+    "tr_data_residual": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
     "tr_data_adjoint": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_int, c_vp]),
     "tr_partial_count": (c_i64, [c_int, c_int, c_int, c_int]),
     "tr_reduce_partials": (c_int, [c_vp, c_i64, c_vp, c_vp]),
@@ -59,6 +60,16 @@ SIGNATURES = {
     "tr_w2_forward": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "tr_w2_adjoint": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "tr_w2_dual_update": (c_int, [c_vp, c_vp, c_i64, c_f32, c_f32, c_vp]),
+    "tr_cnn_conv": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_vp]),
+    "tr_cnn_pack_weights": (c_int, [c_vp, c_int, c_int, c_vp, c_vp]),
+    "tr_cnn_wgrad_work_size": (c_i64, [c_int]),
+    "tr_cnn_wgrad": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "tr_cnn_mse": (c_int, [c_vp, c_vp, c_i64, c_f32, c_vp, c_vp, c_int, c_vp]),
+    "tr_cnn_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_f32, c_int, c_vp]),
+    "tr_cnn_pack_input": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp]),
+    "tr_cnn_unpack": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp]),
+    "tr_cnn_sumsq": (c_int, [c_vp, c_i64, c_int, c_vp, c_vp]),
+    "tr_data_fit_sums": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp]),
 }
 
 _lib = None

@@ -235,17 +235,33 @@ class HipPdhgBase:
             return math.sqrt(max(num, 0.0)) / math.sqrt(den)
         return float("inf")
 
-    def run(self, init=None, progress_every=0):
+    def init_dual_residual(self, x):
+        """q_b = A x_b - y_b: the data dual's fixed point for the primal image x."""
+        s = self.stream
+        for b in range(self.B):
+            _lib.call("tr_data_residual", self.plan.handle, b, x[b].data_ptr(), self.maps.data_ptr(), self.E,
+                      self.C, self.y[b].data_ptr(), self.q[b].data_ptr(), s)
+
+    def run(self, init=None, progress_every=0, dual_init=None):
         """PDHG to the stopping rule; returns ubar (B, E, Z, Y, X) complex64 on the GPU.
 
-        progress_every > 0 prints the iteration and stopping value to stderr
-        every that many iterations.
+        init: starting image (B, E, Z, Y, X) instead of zeros (the reference
+        always starts from zeros).  dual_init="residual" with an init also
+        starts the data dual at its fixed point A init - y; otherwise the data
+        dual starts at the constructor's dual_q (or zero).  progress_every > 0
+        prints the iteration and stopping value to stderr every that many
+        iterations.
         """
         import sys
         import torch
+        if dual_init not in (None, "residual"):
+            raise ValueError(f"unknown dual_init {dual_init!r}")
         with torch.cuda.device(self.device):
             self.pdhg_init(init)
-            self.reset_dual()
+            if init is not None and dual_init == "residual":
+                self.init_dual_residual(self.ubar)
+            else:
+                self.reset_dual()
             torch.cuda.synchronize(self.device)
             t0 = time.perf_counter()
             self.history = []

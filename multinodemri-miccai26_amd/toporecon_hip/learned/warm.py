@@ -1,0 +1,100 @@
+"""CNN warm start for a HIP PDHG solver.
+
+For every (bin, echo) image the network sees the density-compensated adjoint
+g = A^H (dcf * y) divided by its RMS s and predicts the converged image up to
+scale, v = s (x + f(x)).  The scale is then fitted to the data: per (b, e),
+
+    c = Re <A v, y>_dcf / ||A v||^2_dcf      (float64 sums on the GPU)
+
+and the warm start is u0 = c v.  PDHG then runs from u0 to the same
+tolerance as the plain solve, so the network only changes where the
+iterations start, never what they converge to; with no weights, or a shape
+the network was not built for, the caller runs plain PDHG.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _lib
+
+FIT_BLOCKS = 512
+
+
+def dc_adjoint(solver, out=None):
+    """g_b = A^H (dcf * y_b) for every bin, (B, E, Z, Y, X) complex64."""
+    import torch
+    if out is None:
+        out = torch.empty(solver.vol_dims(), dtype=torch.complex64, device=solver.device)
+    s = solver.stream
+    for b in range(solver.B):
+        _lib.call("tr_data_adjoint", solver.plan.handle, b, solver.y[b].data_ptr(), solver.dcf[b].data_ptr(),
+                  solver.maps.data_ptr(), solver.E, solver.C, out[b].data_ptr(), 0, s)
+    return out
+
+
+def image_rms(img, n, V, device, stream):
+    """Per-image RMS (float32, computed in float64) of n images of V voxels."""
+    import torch
+    ss = torch.empty(n, dtype=torch.float64, device=device)
+    _lib.call("tr_cnn_sumsq", img.data_ptr(), V, n, ss.data_ptr(), stream)
+    return torch.sqrt(ss / V).to(torch.float32)
+
+
+def predict(net, img, shape, chunk=6):
+    """Network prediction v = s (x + f(x)) for images (n, Z, Y, X) complex64; returns (v, x, out, s)."""
+    import torch
+    n = img.shape[0]
+    V = int(np.prod(shape))
+    dev = img.device
+    s = net.stream
+    scale = image_rms(img, n, V, dev, s)
+    x = torch.empty((n, *shape, 8), dtype=torch.bfloat16, device=dev)
+    _lib.call("tr_cnn_pack_input", img.data_ptr(), scale.data_ptr(), V, n, x.data_ptr(), s)
+    out = torch.empty((n, *shape, 2), dtype=torch.float32, device=dev)
+    for i0 in range(0, n, chunk):
+        i1 = min(n, i0 + chunk)
+        out[i0:i1] = net.forward(x[i0:i1], i1 - i0, shape)
+    v = torch.empty_like(img)
+    _lib.call("tr_cnn_unpack", x.data_ptr(), out.data_ptr(), scale.data_ptr(), V, n, v.data_ptr(), s)
+    return v, x, out, scale
+
+
+def fit_scales(solver, v):
+    """Least-squares scale per (bin, echo) of v (B, E, Z, Y, X) against the data (float64)."""
+    import torch
+    s = solver.stream
+    K = solver.E * solver.C
+    c = np.zeros((solver.B, solver.E))
+    work = torch.empty((FIT_BLOCKS, K, 2), dtype=torch.float64, device=solver.device)
+    sums = torch.empty((K, 2), dtype=torch.float64, device=solver.device)
+    for b in range(solver.B):
+        M = solver.y[b].shape[0]
+        if M == 0:
+            continue
+        av = torch.empty_like(solver.y[b])
+        _lib.call("tr_data_simulate", solver.plan.handle, b, v[b].data_ptr(), solver.maps.data_ptr(), solver.E,
+                  solver.C, av.data_ptr(), s)
+        _lib.call("tr_data_fit_sums", av.data_ptr(), solver.y[b].data_ptr(), solver.dcf[b].data_ptr(), M, K,
+                  work.data_ptr(), FIT_BLOCKS, sums.data_ptr(), s)
+        h = sums.cpu().numpy().reshape(solver.E, solver.C, 2).sum(axis=1)
+        c[b] = np.where(h[:, 1] > 0, h[:, 0] / np.where(h[:, 1] > 0, h[:, 1], 1.0), 0.0)
+        del av
+    return c
+
+
+def cnn_warm_start(solver, net, chunk=6):
+    """u0 (B, E, Z, Y, X) complex64 for ``solver.run(init=u0, dual_init="residual")``."""
+    import torch
+    shape = tuple(solver.img_shape)
+    B, E = solver.B, solver.E
+    V = int(np.prod(shape))
+    g = dc_adjoint(solver)
+    imgs = g.reshape(B * E, *shape)
+    v, x, out, scale = predict(net, imgs, shape, chunk=chunk)
+    del g
+    c = fit_scales(solver, v.reshape(B, E, *shape))
+    scale2 = scale * torch.from_numpy(c.reshape(-1).astype(np.float32)).to(scale.device)
+    _lib.call("tr_cnn_unpack", x.data_ptr(), out.data_ptr(), scale2.data_ptr(), V, B * E, v.data_ptr(),
+              net.stream)
+    return v.reshape(B, E, *shape)
