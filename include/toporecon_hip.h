@@ -66,6 +66,13 @@ int tr_plan_release_workspace(tr_plan* plan);
  * round-robin; defaults 64 for the interpolation, 0 for the spreading). */
 #define TR_PARAM_INTERP_XCD_CHUNK 1
 #define TR_PARAM_SPREAD_XCD_CHUNK 2
+/* TR_PARAM_PRUNE (default 1): the line passes and the spreading skip grid points outside the
+ * k-space support of the plan's samples (never read by the interpolation, never written by
+ * the spreading); results are identical either way.  TR_PARAM_SUPPORT_POINTS and
+ * TR_PARAM_LIVE_COLUMNS are read-only: the support's size in grid points and (y, x) columns. */
+#define TR_PARAM_PRUNE 3
+#define TR_PARAM_SUPPORT_POINTS 4
+#define TR_PARAM_LIVE_COLUMNS 5
 int tr_plan_set_param(tr_plan* plan, int param, int64_t value);
 int tr_plan_get_param(const tr_plan* plan, int param, int64_t* value);
 

@@ -206,7 +206,8 @@ __device__ __forceinline__ void cl_store(const SpreadArgs& a, int64_t cl, int k,
     const int z = cz * kClZ + pt / (kClY * kClX);
     const int y = cy * kClY + (pt / kClX) % kClY;
     const int x = cx * kClX + pt % kClX;
-    if (z < gz && y < gy && x < gx) a.grid[(((int64_t)z * gy + y) * gx + x) * a.K + k] = acc[i];
+    if (z < gz && y < gy && x < gx && (!a.col_win || in_window(z, a.col_win[y * gx + x], gz)))
+      a.grid[(((int64_t)z * gy + y) * gx + x) * a.K + k] = acc[i];
   }
 }
 
@@ -290,7 +291,8 @@ __global__ void __launch_bounds__(256) k_spread_combine(SpreadArgs a) {
   const int y = cy * kClY + (pt / kClX) % kClY;
   const int x = cx * kClX + pt % kClX;
   const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
-  if (z < gz && y < gy && x < gx) a.grid[(((int64_t)z * gy + y) * gx + x) * K + k] = acc;
+  if (z < gz && y < gy && x < gx && (!a.col_win || in_window(z, a.col_win[y * gx + x], gz)))
+    a.grid[(((int64_t)z * gy + y) * gx + x) * K + k] = acc;
 }
 
 int launch_interp(const InterpArgs& a, hipStream_t s) {
@@ -330,7 +332,8 @@ int launch_interp(const InterpArgs& a, hipStream_t s) {
 
 int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
   const int K = a.K;
-  const double G = (double)p->G;
+  a.col_win = p->prune ? p->col_win : nullptr;
+  const double G = a.col_win ? (double)p->support_points : (double)p->G;
   ProfScope ps(kKSpread, 8.0 * K * a.M + 16.0 * a.M + 8.0 * K * G, s);
   const int kci = K >= 64 ? 64 : pow2ceil(K);
   a.gg = geom(p);

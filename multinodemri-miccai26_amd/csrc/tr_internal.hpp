@@ -91,6 +91,16 @@ struct tr_plan {
   // one XCD by the gather kernels (0 = hardware round-robin order)
   int interp_xcd_chunk = 64;
   int spread_xcd_chunk = 0;
+  // k-space support of all bins' taps: per (y, x) column of the oversampled
+  // grid (natural order) the window (start, len) of z positions any tap
+  // touches; len 0 = empty column.  Grid points outside it are never read by
+  // the interpolation and never receive a spread contribution, so the line
+  // passes skip them.
+  int2* col_win = nullptr;
+  int2* ywin = nullptr;  // per x: the y window covering every live column (y, x)
+  int prune = 1;
+  int64_t live_cols = 0;
+  int64_t support_points = 0;
 };
 
 namespace tr {
@@ -132,6 +142,12 @@ struct LinesArgs {
   int load_n;   // < L: windowed load (positions (s - n//2) mod L), zeros elsewhere
   int store_n;  // < L: windowed store
   int inverse;
+  // support pruning (null = off): line_win[o * win_so + in * win_si] is the
+  // line's window along the FFT axis; only positions inside it are stored
+  // (forward) or loaded (inverse; others are zero), and a line with an empty
+  // window is skipped.
+  const int2* line_win;
+  int win_so, win_si;
 };
 
 struct AdjXArgs {
@@ -197,7 +213,15 @@ struct SpreadArgs {
   const int* heavy_seg_ptr;
   float2* partial;  // nseg * kClPts * K
   int ngrp;         // channel groups per cluster
+  const int2* col_win;  // support windows (null = write every point)
 };
+
+// Position p of an axis of length g lies in the window (start, len) of that axis.
+__device__ __forceinline__ bool in_window(int p, int2 w, int g) {
+  int d = p - w.x;
+  d = d < 0 ? d + g : d;
+  return d < w.y;
+}
 
 // Loads of a line tile are staged in registers (all of a thread's loads are
 // issued before the first LDS write) so that every wave keeps up to
@@ -252,6 +276,7 @@ inline GridGeom geom(const tr_plan* p) {
 
 int ensure_partial(tr_plan* p, int64_t n);
 int ensure_work(tr_plan* p, int64_t K);
+int build_support(tr_plan* p, hipStream_t s);
 int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s);
 void free_plan(tr_plan* p);
 int check_plan_bin(const tr_plan* p, int bin);

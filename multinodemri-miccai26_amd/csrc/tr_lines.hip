@@ -24,6 +24,14 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
   const int chunk = blockIdx.x - line * nchunk;
   const int o = line / a.n_inner;
   const int in = line - o * a.n_inner;
+  // support pruning acts on the k-space side: the stores of a forward pass,
+  // the loads of an inverse one
+  int2 lw = make_int2(0, 0);
+  if (a.line_win) {
+    lw = a.line_win[o * a.win_so + in * a.win_si];
+    if (lw.y == 0) return;  // empty line: nothing live to load or store
+  }
+  auto live_pos = [&](int p) { return !a.line_win || in_window(p, lw, LC > 0 ? LC : a.ax.L); };
   const int ob = a.outer_win_n > 0 ? pmod(o - a.outer_win_n / 2, a.outer_win_g) : o;
   const int k0 = chunk * KC;
   const bool live = k0 + ch < a.K;
@@ -37,7 +45,7 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
     v[it] = make_float2(0.f, 0.f);
     if (sidx < a.load_n && live) {
       const int p = win_load ? pmod(sidx - a.load_n / 2, L) : sidx;
-      v[it] = base[p * a.pos_stride];
+      if (!a.inverse || live_pos(p)) v[it] = base[p * a.pos_stride];
     }
   }
 #pragma unroll
@@ -62,7 +70,7 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
       const int p = win_store ? pmod(sidx - a.store_n / 2, L) : sidx;
       float2 x = buf[p * T::ld + ch];
       if (a.inverse) x.y = -x.y;
-      base[p * a.pos_stride] = x;
+      if (a.inverse || live_pos(p)) base[p * a.pos_stride] = x;
     }
   }
 }

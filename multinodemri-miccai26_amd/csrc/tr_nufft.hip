@@ -8,9 +8,13 @@
 
 namespace tr {
 
-// Forward passes x (with pad), y, z into p->work for K = E*C channels.
+// Forward passes x (with pad), y, z into p->work for K = E*C channels.  With
+// prune, grid points outside the plan's sample support are left unwritten
+// (nothing reads them).
 static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, int E, int C,
-                           hipStream_t s) {
+                           hipStream_t s, bool prune = true) {
+  prune = prune && p->prune && p->col_win;
+  prune = prune && p->prune && p->col_win;
   const int K = E * C;
   int r = ensure_work(p, K);
   if (r) return r;
@@ -52,8 +56,12 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     a.load_n = ny;
     a.store_n = gy;
     a.inverse = 0;
+    a.line_win = prune ? p->ywin : nullptr;
+    a.win_so = 0;
+    a.win_si = 1;
+    const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     dim3 grid((unsigned)(nz * gx * nchunk));
-    ProfScope ps(kKFftYFwd, 8.0 * K * nz * gx * (ny + gy), s);
+    ProfScope ps(kKFftYFwd, 8.0 * K * nz * ((double)gx * ny + cols), s);
     if (int r = launch_lines(kc, gy, grid, fft_smem(gy, kc), s, a)) return r;
   }
   {  // z pass over every (y, x) line, windowed load along z
@@ -71,8 +79,13 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     a.load_n = nz;
     a.store_n = gz;
     a.inverse = 0;
+    a.line_win = prune ? p->col_win : nullptr;
+    a.win_so = gx;
+    a.win_si = 1;
+    const double cols = prune ? (double)p->live_cols : (double)gy * gx;
+    const double pts = prune ? (double)p->support_points : (double)p->G;
     dim3 grid((unsigned)(gy * gx * nchunk));
-    ProfScope ps(kKFftZFwd, 8.0 * K * gy * gx * (nz + gz), s);
+    ProfScope ps(kKFftZFwd, 8.0 * K * (cols * nz + pts), s);
     if (int r = launch_lines(kc, gz, grid, fft_smem(gz, kc), s, a)) return r;
   }
   return kOk;
@@ -80,7 +93,8 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
 
 // Inverse passes z, y (cropping) on p->work, then x with crop/coil sum into out.
 static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* out, int accumulate,
-                         hipStream_t s) {
+                         hipStream_t s, bool prune = true) {
+  prune = prune && p->prune && p->col_win;
   const int K = E * C;
   const int nz = p->n[0], ny = p->n[1];
   const int gz = p->g[0], gy = p->g[1], gx = p->g[2];
@@ -101,8 +115,13 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     a.load_n = gz;
     a.store_n = nz;
     a.inverse = 1;
+    a.line_win = prune ? p->col_win : nullptr;
+    a.win_so = gx;
+    a.win_si = 1;
+    const double cols = prune ? (double)p->live_cols : (double)gy * gx;
+    const double pts = prune ? (double)p->support_points : (double)p->G;
     dim3 grid((unsigned)(gy * gx * nchunk));
-    ProfScope ps(kKFftZInv, 8.0 * K * gy * gx * (nz + gz), s);
+    ProfScope ps(kKFftZInv, 8.0 * K * (cols * nz + pts), s);
     if (int r = launch_lines(kc, gz, grid, fft_smem(gz, kc), s, a)) return r;
   }
   {
@@ -120,8 +139,12 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     a.load_n = gy;
     a.store_n = ny;
     a.inverse = 1;
+    a.line_win = prune ? p->ywin : nullptr;
+    a.win_so = 0;
+    a.win_si = 1;
+    const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     dim3 grid((unsigned)(nz * gx * nchunk));
-    ProfScope ps(kKFftYInv, 8.0 * K * nz * gx * (ny + gy), s);
+    ProfScope ps(kKFftYInv, 8.0 * K * nz * ((double)gx * ny + cols), s);
     if (int r = launch_lines(kc, gy, grid, fft_smem(gy, kc), s, a)) return r;
   }
   {
@@ -200,7 +223,7 @@ int tr_nufft_forward_grid(tr_plan* p, const tr_c64* img, const tr_c64* maps, int
   if (!p) TR_FAIL(kInvalidArgument, "null plan");
   if (E < 1 || C < 1 || (!maps && C != 1)) TR_FAIL(kInvalidArgument, "need E >= 1 and C == 1 without maps");
   hipStream_t s = (hipStream_t)stream;
-  if (int r = forward_to_grid(p, F2(img), F2(maps), E, C, s)) return r;
+  if (int r = forward_to_grid(p, F2(img), F2(maps), E, C, s, false)) return r;
   TR_HIP(hipMemcpyAsync(grid, p->work, sizeof(float2) * p->G * E * C, hipMemcpyDeviceToDevice, s));
   return kOk;
 }
@@ -212,7 +235,7 @@ int tr_nufft_adjoint_grid(tr_plan* p, const tr_c64* grid, const tr_c64* maps, in
   hipStream_t s = (hipStream_t)stream;
   if (int r = ensure_work(p, (int64_t)E * C)) return r;
   TR_HIP(hipMemcpyAsync(p->work, grid, sizeof(float2) * p->G * E * C, hipMemcpyDeviceToDevice, s));
-  return grid_to_image(p, F2(maps), E, C, F2(out), 0, s);
+  return grid_to_image(p, F2(maps), E, C, F2(out), 0, s, false);
 }
 
 int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* maps, int E, int C,

@@ -134,6 +134,35 @@ __global__ void k_gather_pairs(const ClusterPair* in, const int* idx, int64_t nn
   if (i < nnz) out[i] = in[idx[i]];
 }
 
+// Per (y, x) column of the oversampled grid (natural order), the smallest and
+// largest SigPy grid index along z that any tap of any sample touches.
+__global__ void k_support_minmax(const float4* kp, int64_t M, int3 g, int* lo, int* hi) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const float4 k = kp[j];
+  const float kk[3] = {k.x, k.y, k.z};
+  int t0[3], n[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!isfinite(kk[ax]) || fabsf(kk[ax]) > 1.0e6f) return;  // no taps (as axis_taps)
+    const double kd = (double)kk[ax];
+    t0[ax] = (int)ceil(kd - 2.0);
+    n[ax] = (int)floor(kd + 2.0) - t0[ax] + 1;
+  }
+  for (int dy = 0; dy < n[1]; ++dy) {
+    const int py = pmod(t0[1] + dy - g.y / 2, g.y);
+    for (int dx = 0; dx < n[2]; ++dx) {
+      const int col = py * g.z + pmod(t0[2] + dx - g.z / 2, g.z);
+      atomicMin(&lo[col], t0[0]);
+      atomicMax(&hi[col], t0[0] + n[0] - 1);
+    }
+  }
+}
+
+__global__ void k_fill_int(int* a, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 __global__ void k_row_ptr(const unsigned* key, int64_t nnz, int64_t R, int* row_ptr) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t > R) return;
@@ -321,6 +350,83 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   return kOk;
 }
 
+// Sample support of all bins -> p->col_win (natural z window per column).
+int build_support(tr_plan* p, hipStream_t s) {
+  const int gz = p->g[0], gy = p->g[1], gx = p->g[2];
+  const int64_t ncol = (int64_t)gy * gx;
+  int *lo = nullptr, *hi = nullptr;
+  TR_HIP(hipMalloc(&lo, sizeof(int) * ncol));
+  TR_HIP(hipMalloc(&hi, sizeof(int) * ncol));
+  hipLaunchKernelGGL(k_fill_int, dim3(grid1d(ncol, 256)), dim3(256), 0, s, lo, ncol, 0x7fffffff);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid1d(ncol, 256)), dim3(256), 0, s, hi, ncol, (int)0x80000000);
+  TR_LAUNCH_CHECK();
+  const int3 g3 = make_int3(gz, gy, gx);
+  for (const BinData& b : p->bins) {
+    if (b.M == 0) continue;
+    hipLaunchKernelGGL(k_support_minmax, dim3(grid1d(b.M, 256)), dim3(256), 0, s, b.kp, b.M, g3, lo, hi);
+    TR_LAUNCH_CHECK();
+  }
+  std::vector<int> hlo(ncol), hhi(ncol);
+  TR_HIP(hipMemcpyAsync(hlo.data(), lo, sizeof(int) * ncol, hipMemcpyDeviceToHost, s));
+  TR_HIP(hipMemcpyAsync(hhi.data(), hi, sizeof(int) * ncol, hipMemcpyDeviceToHost, s));
+  TR_HIP(hipStreamSynchronize(s));
+  TR_HIP(hipFree(lo));
+  TR_HIP(hipFree(hi));
+  std::vector<int2> win(ncol);
+  for (int64_t c = 0; c < ncol; ++c) {
+    if (hhi[c] < hlo[c]) {
+      win[c] = make_int2(0, 0);
+      continue;
+    }
+    const int64_t len = (int64_t)hhi[c] - hlo[c] + 1;
+    win[c] = len >= gz ? make_int2(0, gz) : make_int2(pmod(hlo[c] - gz / 2, gz), (int)len);
+  }
+  // The y passes take one window per x: the shortest circular window holding
+  // every live column (y, x) (the complement of the longest run of dead ones).
+  // A dead column inside it gets a one-point z window, which the spreading
+  // fills with zero, so the inverse y pass only reads written points.
+  std::vector<int2> yw(gx);
+  for (int x = 0; x < gx; ++x) {
+    int nlive = 0;
+    for (int y = 0; y < gy; ++y) nlive += win[(int64_t)y * gx + x].y > 0;
+    if (nlive == 0) {
+      yw[x] = make_int2(0, 0);
+      continue;
+    }
+    if (nlive == gy) {
+      yw[x] = make_int2(0, gy);
+      continue;
+    }
+    int best_len = 0, best_end = 0, run = 0;
+    for (int i = 0; i < 2 * gy; ++i) {  // longest circular run of dead columns
+      if (win[(int64_t)(i % gy) * gx + x].y == 0) {
+        if (++run > best_len && run <= gy) {
+          best_len = run;
+          best_end = i % gy;
+        }
+      } else {
+        run = 0;
+      }
+    }
+    yw[x] = make_int2(pmod(best_end + 1, gy), gy - best_len);
+    for (int i = 0; i < yw[x].y; ++i) {
+      int2& w = win[(int64_t)((yw[x].x + i) % gy) * gx + x];
+      if (w.y == 0) w = make_int2(0, 1);
+    }
+  }
+  p->live_cols = 0;
+  p->support_points = 0;
+  for (int64_t c = 0; c < ncol; ++c) {
+    p->live_cols += win[c].y > 0;
+    p->support_points += win[c].y;
+  }
+  TR_HIP(hipMalloc(&p->ywin, sizeof(int2) * gx));
+  TR_HIP(hipMemcpy(p->ywin, yw.data(), sizeof(int2) * gx, hipMemcpyHostToDevice));
+  TR_HIP(hipMalloc(&p->col_win, sizeof(int2) * ncol));
+  TR_HIP(hipMemcpy(p->col_win, win.data(), sizeof(int2) * ncol, hipMemcpyHostToDevice));
+  return kOk;
+}
+
 void free_plan(tr_plan* p) {
   if (!p) return;
   for (auto& b : p->bins) {
@@ -340,6 +446,8 @@ void free_plan(tr_plan* p) {
   }
   (void)hipFree(p->work);
   (void)hipFree(p->partial);
+  (void)hipFree(p->col_win);
+  (void)hipFree(p->ywin);
   delete p;
 }
 
@@ -421,6 +529,12 @@ int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float
       return r;
     }
   }
+  if (const int r = build_support(p, s)) {
+    std::string msg = last_error();
+    free_plan(p);
+    set_error(msg);
+    return r;
+  }
   *out = p;
   return kOk;
 }
@@ -463,10 +577,19 @@ int tr_plan_release_workspace(tr_plan* p) {
 
 int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
   if (!p) TR_FAIL(kInvalidArgument, "null plan");
-  if (value < 0 || value > 4096) TR_FAIL(kInvalidArgument, "parameter value outside [0, 4096]");
   switch (param) {
-    case TR_PARAM_INTERP_XCD_CHUNK: p->interp_xcd_chunk = (int)value; return kOk;
-    case TR_PARAM_SPREAD_XCD_CHUNK: p->spread_xcd_chunk = (int)value; return kOk;
+    case TR_PARAM_INTERP_XCD_CHUNK:
+    case TR_PARAM_SPREAD_XCD_CHUNK:
+      if (value < 0 || value > 4096) TR_FAIL(kInvalidArgument, "parameter value outside [0, 4096]");
+      (param == TR_PARAM_INTERP_XCD_CHUNK ? p->interp_xcd_chunk : p->spread_xcd_chunk) = (int)value;
+      return kOk;
+    case TR_PARAM_PRUNE:
+      if (value != 0 && value != 1) TR_FAIL(kInvalidArgument, "prune is 0 or 1");
+      p->prune = (int)value;
+      return kOk;
+    case TR_PARAM_SUPPORT_POINTS:
+    case TR_PARAM_LIVE_COLUMNS:
+      TR_FAIL(kInvalidArgument, "read-only plan parameter " + std::to_string(param));
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));
   }
 }
@@ -476,6 +599,9 @@ int tr_plan_get_param(const tr_plan* p, int param, int64_t* value) {
   switch (param) {
     case TR_PARAM_INTERP_XCD_CHUNK: *value = p->interp_xcd_chunk; return kOk;
     case TR_PARAM_SPREAD_XCD_CHUNK: *value = p->spread_xcd_chunk; return kOk;
+    case TR_PARAM_PRUNE: *value = p->prune; return kOk;
+    case TR_PARAM_SUPPORT_POINTS: *value = p->support_points; return kOk;
+    case TR_PARAM_LIVE_COLUMNS: *value = p->live_cols; return kOk;
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));
   }
 }

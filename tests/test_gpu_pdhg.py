@@ -146,3 +146,24 @@ def test_residual_dual_init_matches_definition(torch_mod):
         _lib.call("tr_data_simulate", solver.plan.handle, b, x[b].data_ptr(), solver.maps.data_ptr(),
                   solver.E, solver.C, ax.data_ptr(), solver.stream)
         assert torch.equal(solver.q[b], ax - solver.y[b])
+
+
+@pytest.mark.gpu
+def test_support_pruning_changes_no_bit(torch_mod):
+    """The line passes and the spreading skip grid points outside the samples' k-space
+    support; the cone trajectory leaves the grid's corners empty, and skipping them
+    changes no result bit of a solve (forward, adjoint and the PDHG iterates)."""
+    acq, ksp, prob = make_problem(seed=13)
+    solver = _hip_solver("tvme", acq, ksp, 4, 0.0, True)
+    plan = solver.plan
+    G = int(np.prod(plan.grid_shape))
+    pts, cols = plan.get_param("support_points"), plan.get_param("live_columns")
+    assert 0 < pts < G and 0 < cols <= plan.grid_shape[1] * plan.grid_shape[2]
+    assert plan.get_param("prune") == 1
+    pruned = solver.run().cpu().numpy()
+    plan.set_param("prune", 0)
+    full = solver.run().cpu().numpy()
+    plan.set_param("prune", 1)
+    assert np.array_equal(pruned.view(np.uint64), full.view(np.uint64))
+    with pytest.raises(Exception):
+        plan.set_param("support_points", 5)
