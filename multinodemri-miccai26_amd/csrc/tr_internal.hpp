@@ -148,6 +148,7 @@ struct LinesArgs {
   // window is skipped.
   const int2* line_win;
   int win_so, win_si;
+  int zero_empty;  // an empty line still stores zeros (its outputs are read later)
 };
 
 struct AdjXArgs {

@@ -29,7 +29,22 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
   int2 lw = make_int2(0, 0);
   if (a.line_win) {
     lw = a.line_win[o * a.win_so + in * a.win_si];
-    if (lw.y == 0) return;  // empty line: nothing live to load or store
+    if (lw.y == 0) {  // empty line: nothing live to load; its outputs are zero
+      if (a.zero_empty) {
+        const int nchunk0 = (a.K + KC - 1) / KC;
+        const int k00 = (blockIdx.x % nchunk0) * KC;
+        if (k00 + ch < a.K) {
+          const int Ls = LC > 0 ? LC : a.ax.L;
+          const int ob0 = a.outer_win_n > 0 ? pmod(o - a.outer_win_n / 2, a.outer_win_g) : o;
+          float2* b0 = a.data + ob0 * a.outer_stride + in * a.inner_stride + k00 + ch;
+          for (int sidx = w; sidx < a.store_n; sidx += T::W) {
+            const int p = a.store_n < Ls ? pmod(sidx - a.store_n / 2, Ls) : sidx;
+            b0[p * a.pos_stride] = make_float2(0.f, 0.f);
+          }
+        }
+      }
+      return;
+    }
   }
   auto live_pos = [&](int p) { return !a.line_win || in_window(p, lw, LC > 0 ? LC : a.ax.L); };
   const int ob = a.outer_win_n > 0 ? pmod(o - a.outer_win_n / 2, a.outer_win_g) : o;

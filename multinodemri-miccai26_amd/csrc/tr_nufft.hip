@@ -59,6 +59,7 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     a.line_win = prune ? p->ywin : nullptr;
     a.win_so = 0;
     a.win_si = 1;
+    a.zero_empty = 0;
     const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     dim3 grid((unsigned)(nz * gx * nchunk));
     ProfScope ps(kKFftYFwd, 8.0 * K * nz * ((double)gx * ny + cols), s);
@@ -82,6 +83,7 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     a.line_win = prune ? p->col_win : nullptr;
     a.win_so = gx;
     a.win_si = 1;
+    a.zero_empty = 0;  // the y pass reads only columns inside its windows
     const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     const double pts = prune ? (double)p->support_points : (double)p->G;
     dim3 grid((unsigned)(gy * gx * nchunk));
@@ -118,6 +120,7 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     a.line_win = prune ? p->col_win : nullptr;
     a.win_so = gx;
     a.win_si = 1;
+    a.zero_empty = 0;  // the y pass reads only columns inside its windows
     const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     const double pts = prune ? (double)p->support_points : (double)p->G;
     dim3 grid((unsigned)(gy * gx * nchunk));
@@ -142,6 +145,7 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     a.line_win = prune ? p->ywin : nullptr;
     a.win_so = 0;
     a.win_si = 1;
+    a.zero_empty = 1;  // the x pass reads every (z, y, kx) it leaves
     const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     dim3 grid((unsigned)(nz * gx * nchunk));
     ProfScope ps(kKFftYInv, 8.0 * K * nz * ((double)gx * ny + cols), s);
