@@ -56,6 +56,10 @@ def build_parser(algorithm: str) -> argparse.ArgumentParser:
         p.add_argument("--reduce-per-bin", "--reduce_per_bin", dest="reduce_per_bin", action="store_true")
     p.add_argument("--fp16-communication", "--use_fp16_comm", dest="use_fp16_comm", action="store_true")
     p.add_argument("--nufft-backend", choices=("sigpy", "hip"), default="hip")
+    if algorithm == "tvme":
+        # not a reference flag: start PDHG from the trained CNN's estimate (same tolerance)
+        p.add_argument("--warm-start", choices=("none", "learned"), default="none")
+        p.add_argument("--warm-start-weights", type=str, default=None)
     p.add_argument("--prepared-dir", type=str, default=None)
     p.add_argument("--output-dir", type=str, default=".")
     p.add_argument("--verbose", action="store_true")
@@ -142,6 +146,8 @@ def _manifest(args, algorithm, E_total, prepared):
         density = "maximum_then_global_maximum_after_motion_sharding"
     elif algorithm == "tvme":
         params.update(lambda_motion=args.lamda_m, lambda_echo=args.lamda_e, acceleration=args.acc)
+        if getattr(args, "warm_start", "none") != "none":  # absent in the reference's manifest
+            params.update(warm_start=args.warm_start)
         density = "maximum"
     else:
         params.update(lambda_motion=args.lambda1, lambda_echo_wavelet=args.lambda2,
@@ -193,7 +199,15 @@ def run_main(algorithm: str, argv=None) -> int:
         rec = TvmwReconstructor(ksp, coord, dcf, mps, resp, None, B, args.l2_coupling, lambda1=args.lambda1,
                                 lambda2=args.lambda2, lambda3=args.lambda3, E_total=data["E_total"], e0=0,
                                 e1=data["E_total"], **common)
-    img = rec.run().cpu().numpy()
+    if getattr(args, "warm_start", "none") == "learned":
+        from .learned import WarmStartNet, cnn_warm_start, default_weights_path
+        path = args.warm_start_weights or default_weights_path()
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"--warm-start learned: no weights at {path}")
+        net = WarmStartNet.load(path, args.device)
+        img = rec.run(init=cnn_warm_start(rec, net), dual_init="residual").cpu().numpy()
+    else:
+        img = rec.run().cpu().numpy()
     logging.info("Reconstruction done in %.2f seconds (%d iterations).", time.perf_counter() - t0,
                  rec.iterations)
     Bn, E = img.shape[:2]

@@ -64,3 +64,31 @@ def test_module_cli_forwards_arguments(torch_mod, tmp_path):
     assert (out / "img_e0-1_m0-2.cfl").exists()
     with pytest.raises(RuntimeError, match="SigPy is required"):
         run_main("tvm", [str(tmp_path / "raw"), "img", "--nufft-backend", "sigpy"])
+
+
+def test_tvme_cli_learned_warm_start(torch_mod, tmp_path):
+    """--warm-start learned (TVME): same shard format, the manifest says so, and the image
+    agrees with the plain solve to the stopping tolerance's accuracy."""
+    _write_dataset(tmp_path / "raw", "tvme")
+    base = [str(tmp_path / "raw"), "img", "--num-bins", "3", "--max-iter", "200", "--tol", "1e-3",
+            "--l2-coupling", "--lamda_m", "2e-3", "--lamda_e", "2e-3"]
+    assert run_main("tvme", base + ["--output-dir", str(tmp_path / "plain")]) == 0
+    assert run_main("tvme", base + ["--output-dir", str(tmp_path / "warm"), "--warm-start", "learned"]) == 0
+    plain = cfl.read_cfl(tmp_path / "plain" / "img_e0-1_m0-2")
+    warm = cfl.read_cfl(tmp_path / "warm" / "img_e0-1_m0-2")
+    assert warm.shape == plain.shape
+    # both stop on the relative-change rule; compare what they minimise (float64 oracle objective)
+    args = build_parser("tvme").parse_args(base)
+    d = load_conditioned(args, "tvme")
+    prob = op.bin_problem(d["ksp"], d["coord"], d["dcf"], d["mps"], d["resp"], d["bin_edges"])
+    kw = dict(lam_m=2e-3, lam_e=2e-3, l2_coupling=True)
+    phi_p = op.objective(prob, plain[:, 0, 0, :, 0, 0].astype(np.complex128), "tvme", **kw)
+    phi_w = op.objective(prob, warm[:, 0, 0, :, 0, 0].astype(np.complex128), "tvme", **kw)
+    print("objective plain", phi_p, "warm", phi_w, "image rel diff", rel(warm, plain), flush=True)
+    # On this 10x12x14 fixture the relative-change rule stops both runs early, so only a coarse
+    # agreement is asserted here; test_learned_warm_start_gains_over_plain checks the 0.1 %
+    # objective bound on a realistic held-out problem.
+    assert abs(phi_w - phi_p) / phi_p < 3e-2
+    man = json.loads((tmp_path / "warm" / "run_manifest.json").read_text())
+    assert man["parameters"]["warm_start"] == "learned"
+    assert "warm_start" not in json.loads((tmp_path / "plain" / "run_manifest.json").read_text())["parameters"]
