@@ -337,6 +337,12 @@ def main(argv=None):
         del u0, u_warm
         log(f"warm start: {solver.iterations} iterations, {t_warm:.2f} s (start {t_start:.2f} s), objective "
             f"{phi_warm:.6e} ({rel:+.2e} vs plain), speedup {t_plain / t_warm:.2f}x", verbose)
+    # step-size check: float64 power iteration for ||K|| (outside the timed legs)
+    t0 = time.perf_counter()
+    knorm = solver.operator_norm(iters=20)
+    knorm = {k: knorm[k] for k in ("K_norm", "sigma_tau_K_norm_sq", "iterations")}
+    log(f"power iteration: ||K|| = {knorm['K_norm']:.4f}, sigma tau ||K||^2 = "
+        f"{knorm['sigma_tau_K_norm_sq']:.4f} ({time.perf_counter() - t0:.1f} s)", verbose)
     use_warm = warm is not None and warm["within_0.1pct"]
     t_step = warm["time_to_tol_s"] if use_warm else t_plain
 
@@ -393,6 +399,7 @@ def main(argv=None):
                    "plain": {"time_to_tol_s": t_plain, "iterations": n_it, "objective": phi_plain,
                              "s_per_iteration": t_plain / max(n_it, 1)},
                    "learned_warm_start": warm,
+                   "operator_norm": knorm,
                    "iterations": warm["iterations"] if use_warm else n_it,
                    "s_per_iteration": t_plain / max(n_it, 1),
                    "samples_per_bin": samples, "grid": list(solver.plan.grid_shape),

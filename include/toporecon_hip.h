@@ -125,6 +125,8 @@ int tr_data_residual(tr_plan* plan, int bin, const tr_c64* x, const tr_c64* maps
 int tr_data_adjoint(tr_plan* plan, int bin, const tr_c64* q_sorted, const float* dcf_sorted,
                     const tr_c64* maps, int E, int C, tr_c64* g_b, int accumulate, void* stream);
 
+/* out = s * x (complex64, n values): the power iteration's normalisation. */
+int tr_cscale(const tr_c64* x, int64_t n, double s, tr_c64* out, void* stream);
 /* ---- element-wise PDHG steps on (B, E, Z, Y, X) arrays ----
  * `partial` holds tr_partial_count() float64 pairs (sum |u - u_old|^2, sum |u|^2)
  * reduced by tr_reduce_partials into out2[2]. */

@@ -296,6 +296,11 @@ __global__ void __launch_bounds__(kEwBlock) k_tvmw_primal(VolGeom g, float2* u, 
 }
 
 // Fixed-order final reduction of the per-block partial sums (one block).
+__global__ void __launch_bounds__(kEwBlock) k_cscale(const float2* x, int64_t n, float s, float2* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2(s * x[i].x, s * x[i].y);
+}
+
 __global__ void __launch_bounds__(kEwBlock) k_reduce_partials(const double2* partial, int n, double* out) {
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < n; i += kEwBlock) {
@@ -409,6 +414,15 @@ int tr_tvmw_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, c
   ProfScope ps(kKTvmwPrimal, 9.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
   hipLaunchKernelGGL(k_tvmw_primal, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(u),
                      F2(ubar), F2(grad), F2(p_m), F2(p_w1), F2(divw2), tau, (double2*)partial);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_cscale(const tr_c64* x, int64_t n, double s, tr_c64* out, void* stream) {
+  if ((!x || !out) && n > 0) TR_FAIL(kInvalidArgument, "null argument");
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_cscale, dim3((unsigned)((n + kEwBlock - 1) / kEwBlock)), dim3(kEwBlock), 0,
+                     (hipStream_t)stream, (const float2*)x, n, (float)s, (float2*)out);
   TR_LAUNCH_CHECK();
   return kOk;
 }

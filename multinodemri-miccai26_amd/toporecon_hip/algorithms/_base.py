@@ -229,6 +229,66 @@ class HipPdhgBase:
         del img, ksp
         return out
 
+    # -- step-size check ------------------------------------------------------------
+    def _tv_normal(self, x, u, ubar_scr, g, scratch) -> float:
+        """u <- K^H K x from u = 0, given g = A^H W A x: the TV part of K through the
+        algorithm's dual step (sigma = 1, no projection, duals from zero) and its adjoint
+        through the primal step with tau = -1.  Returns sum |u|^2 (float64)."""
+        raise NotImplementedError(f"{type(self).__name__} has no power iteration")
+
+    def operator_norm(self, iters: int = 20, seed: int = 0) -> dict:
+        """Power-iteration estimate of ||K|| for the stacked operator K = [W^1/2 A; TV parts]
+        whose adjoint the primal step applies.  The normal operator K^H K x is built from the
+        solver's own kernels (the data term through the NUFFT, the TV part through the dual step
+        with sigma = 1 and no projection followed by the primal step with tau = -1 from zero),
+        and every norm is a float64 reduction on the GPU.  Returns ||K||, sigma tau ||K||^2
+        (PDHG needs < 1) and the per-iteration estimates."""
+        import torch
+        dims = self.vol_dims()
+        dev = self.device
+        s = self.stream
+        n = int(np.prod(dims))
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(seed))
+        with torch.cuda.device(dev):
+            x = torch.complex(torch.randn(dims, generator=gen, device=dev),
+                              torch.randn(dims, generator=gen, device=dev))
+            u = torch.zeros_like(x)
+            ubar_scr = torch.empty_like(x)
+            g = torch.empty_like(x)
+            qtmp = [torch.empty_like(self.y[b]) for b in range(self.B)]
+            self._alloc_reduction()
+            scratch = {}
+            # normalise the start vector (||x|| from the same float64 reduction)
+            nrm = math.sqrt(self._sumsq(x))
+            _lib.call("tr_cscale", x.data_ptr(), n, 1.0 / nrm, x.data_ptr(), s)
+            hist = []
+            for _ in range(iters):
+                for b in range(self.B):
+                    _lib.call("tr_data_simulate", self.plan.handle, b, x[b].data_ptr(), self.maps.data_ptr(),
+                              self.E, self.C, qtmp[b].data_ptr(), s)
+                    _lib.call("tr_data_adjoint", self.plan.handle, b, qtmp[b].data_ptr(), self.dcf[b].data_ptr(),
+                              self.maps.data_ptr(), self.E, self.C, g[b].data_ptr(), 0, s)
+                u.zero_()
+                num = self._tv_normal(x, u, ubar_scr, g, scratch)  # u = K^H K x, num = ||u||^2
+                lam = math.sqrt(max(num, 0.0))  # ||K^H K x|| with ||x|| = 1
+                hist.append(lam)
+                if lam == 0.0:
+                    break
+                _lib.call("tr_cscale", u.data_ptr(), n, 1.0 / lam, x.data_ptr(), s)
+            del qtmp, g, u, ubar_scr, x, scratch
+        k2 = hist[-1] if hist else 0.0
+        return {"K_norm": math.sqrt(k2), "K_norm_sq": k2, "sigma_tau_K_norm_sq": self.sigma * self.tau * k2,
+                "iterations": len(hist), "history_K_norm_sq": hist}
+
+    def _sumsq(self, x):
+        """sum |x|^2 of a (B, E, Z, Y, X) array, float64 on the GPU (per-image sums added on the host)."""
+        import torch
+        nimg = self.B * self.E
+        out = torch.empty(nimg, dtype=torch.float64, device=self.device)
+        _lib.call("tr_cnn_sumsq", x.data_ptr(), int(np.prod(self.img_shape)), nimg, out.data_ptr(), self.stream)
+        return float(out.sum())
+
     # -- run loop ----------------------------------------------------------------
     def _converged_value(self, num, den):
         if math.isfinite(num) and math.isfinite(den) and den > 0.0:

@@ -46,6 +46,18 @@ class TvmReconstructor(HipPdhgBase):
             self.u.copy_(init)
         self._alloc_reduction()
 
+    def _tv_normal(self, x, u, ubar_scr, g, scratch):
+        dims = self.vol_dims()
+        s = self.stream
+        if "p_m" in scratch:
+            scratch["p_m"].zero_()
+        else:
+            scratch["p_m"] = torch.zeros_like(x)
+        _lib.call("tr_tvm_dual", *dims, x.data_ptr(), scratch["p_m"].data_ptr(), 1.0, 1e30, int(self.l2_coupling), s)
+        _lib.call("tr_tvm_primal", *dims, u.data_ptr(), ubar_scr.data_ptr(), g.data_ptr(), scratch["p_m"].data_ptr(),
+                  -1.0, self._partial.data_ptr(), s)
+        return self._reduce()[0]
+
     def pdhg(self, it):
         """One PDHG iteration (tvm.py:227-380); returns (sum|du|^2, sum|u|^2) in float64."""
         dims = self.vol_dims()

@@ -58,6 +58,20 @@ class TvmeReconstructor(HipPdhgBase):
             self.u.copy_(init)
         self._alloc_reduction()
 
+    def _tv_normal(self, x, u, ubar_scr, g, scratch):
+        dims = self.vol_dims()
+        s = self.stream
+        for k in ("p_m", "p_ex", "p_ey", "p_ez"):
+            if k in scratch:
+                scratch[k].zero_()
+            else:
+                scratch[k] = torch.zeros_like(x)
+        p = [scratch[k].data_ptr() for k in ("p_m", "p_ex", "p_ey", "p_ez")]
+        _lib.call("tr_tvme_dual", *dims, x.data_ptr(), *p, 1.0, 1e30, 1e30, int(self.l2_coupling), s)
+        _lib.call("tr_tvme_primal", *dims, u.data_ptr(), ubar_scr.data_ptr(), g.data_ptr(), *p, -1.0,
+                  self._partial.data_ptr(), s)
+        return self._reduce()[0]
+
     def pdhg(self, it):
         """One PDHG iteration (tvme.py:208-482); returns (sum|du|^2, sum|u|^2) in float64."""
         dims = self.vol_dims()

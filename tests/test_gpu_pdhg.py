@@ -167,3 +167,55 @@ def test_support_pruning_changes_no_bit(torch_mod):
     assert np.array_equal(pruned.view(np.uint64), full.view(np.uint64))
     with pytest.raises(Exception):
         plan.set_param("support_points", 5)
+
+
+def _oracle_normal(prob, x, alg):
+    """K^H K x composed from the oracle's own step pieces (sigma = 1, no prox, tau = -1)."""
+    g = op.adjoint_op(prob, op.forward_op(prob, x))
+    dm = op.motion_diff(x)
+    if alg == "tvm":
+        return g - op.motion_div_tvm(dm)
+    de = op.echo_diff(x)
+    divp = op.fdiff_adj(op.fdiff(de, -1), -1) + op.fdiff_adj(op.fdiff(de, -2), -2) \
+        + op.fdiff_adj(op.fdiff(de, -3), -3)
+    return g + op.motion_div_tvme(dm) + op.echo_div(divp)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg", ["tvm", "tvme"])
+def test_operator_norm_power_iteration(torch_mod, alg):
+    """The GPU normal operator K^H K equals the oracle's, and the float64 power iteration's
+    ||K||^2 agrees with a float64 power iteration on the oracle."""
+    torch = torch_mod
+    from toporecon_hip import _lib
+    acq, ksp, prob = make_problem(seed=14)
+    solver = _hip_solver(alg, acq, ksp, 1, 0.0, True)
+    solver._alloc_reduction()
+    rng = np.random.default_rng(3)
+    dims = solver.vol_dims()
+    xn = (rng.standard_normal(dims) + 1j * rng.standard_normal(dims)).astype(np.complex64)
+    x = torch.from_numpy(xn).cuda()
+    g = torch.empty_like(x)
+    u = torch.zeros_like(x)
+    scr = torch.empty_like(x)
+    q = [torch.empty_like(solver.y[b]) for b in range(solver.B)]
+    for b in range(solver.B):
+        _lib.call("tr_data_simulate", solver.plan.handle, b, x[b].data_ptr(), solver.maps.data_ptr(), solver.E,
+                  solver.C, q[b].data_ptr(), solver.stream)
+        _lib.call("tr_data_adjoint", solver.plan.handle, b, q[b].data_ptr(), solver.dcf[b].data_ptr(),
+                  solver.maps.data_ptr(), solver.E, solver.C, g[b].data_ptr(), 0, solver.stream)
+    num = solver._tv_normal(x, u, scr, g, {})
+    ref = _oracle_normal(prob, xn.astype(np.complex128), alg)
+    assert rel(u.cpu().numpy(), ref) < 2e-5
+    assert abs(num - float(np.sum(np.abs(ref) ** 2))) / float(np.sum(np.abs(ref) ** 2)) < 5e-5
+    est = solver.operator_norm(iters=60, seed=1)
+    v = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
+    v /= np.linalg.norm(v)
+    lam = 0.0
+    for _ in range(60):
+        w = _oracle_normal(prob, v, alg)
+        lam = np.linalg.norm(w)
+        v = w / lam
+    print(alg, "K^2 gpu", est["K_norm_sq"], "oracle", lam, "sigma tau K^2", est["sigma_tau_K_norm_sq"], flush=True)
+    assert abs(est["K_norm_sq"] - lam) / lam < 1e-2
+    assert est["sigma_tau_K_norm_sq"] == pytest.approx(lam / 36.0, rel=1e-2)
