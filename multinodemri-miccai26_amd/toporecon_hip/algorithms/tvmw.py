@@ -94,6 +94,34 @@ class TvmwReconstructor(HipPdhgBase):
         _lib.call("tr_w2_adjoint", c.data_ptr(), B * E, Z, Y, X, self.level_w2, out.data_ptr(),
                   self.w2_work.data_ptr(), self.stream)
 
+    def _tv_normal(self, x, u, ubar_scr, g, scratch):
+        B, E, Z, Y, X = dims = self.vol_dims()
+        s = self.stream
+        if not scratch:
+            scratch["p_m"] = torch.zeros_like(x)
+            scratch["p_w1"] = torch.zeros_like(x)
+            scratch["p_w2"] = torch.zeros((B * E, self.ncoef), dtype=torch.complex64, device=self.device)
+            scratch["w2"] = torch.empty_like(scratch["p_w2"])
+            scratch["divw2"] = torch.empty_like(x)
+            scratch["work"] = torch.empty((max(B * E * self.nwork, 1),), dtype=torch.complex64,
+                                          device=self.device)
+        for k in ("p_m", "p_w1", "p_w2"):
+            scratch[k].zero_()
+        work = scratch["work"].data_ptr()
+        _lib.call("tr_w2_forward", x.data_ptr(), B * E, Z, Y, X, self.level_w2, scratch["w2"].data_ptr(), work, s)
+        _lib.call("tr_w2_dual_update", scratch["p_w2"].data_ptr(), scratch["w2"].data_ptr(),
+                  scratch["p_w2"].numel(), 1.0, 1e30, s)
+        _lib.call("tr_tvmw_dual", *dims, x.data_ptr(), scratch["p_m"].data_ptr(), scratch["p_w1"].data_ptr(),
+                  1.0, 1e30, 1e30, int(self.l2_coupling), s)
+        _lib.call("tr_w2_adjoint", scratch["p_w2"].data_ptr(), B * E, Z, Y, X, self.level_w2,
+                  scratch["divw2"].data_ptr(), work, s)
+        ubar_scr.zero_()
+        _lib.call("tr_tvmw_primal", *dims, u.data_ptr(), ubar_scr.data_ptr(), g.data_ptr(),
+                  scratch["p_m"].data_ptr(), scratch["p_w1"].data_ptr(), scratch["divw2"].data_ptr(), -1.0,
+                  self._partial.data_ptr(), s)
+        # the TVMW stop sums compare ubar before (0) and after (2 u): num = 4 |u|^2
+        return self._reduce()[0] / 4.0
+
     def pdhg(self, it):
         """One PDHG iteration (tvmw.py:385-685); returns (sum|dubar|^2, sum|ubar_prev|^2)."""
         dims = self.vol_dims()
