@@ -25,6 +25,7 @@ extern "C" {
 #define TR_ABI_VERSION 1
 
 typedef struct tr_plan tr_plan;
+typedef struct tr_graph tr_graph;
 
 /* complex64 as stored by NumPy / PyTorch: real then imaginary float32. */
 typedef struct tr_c64 {
@@ -45,6 +46,16 @@ int tr_profile_enable(int on);
 int tr_profile_num_kernels(void);
 const char* tr_profile_kernel_name(int id);
 int tr_profile_read(int id, double* ms_total, int64_t* launches, double* bytes_total);
+
+/* ---- HIP graph capture (no reference counterpart: the reference launches every kernel from
+ * Python each iteration).  Everything issued to `stream` (and to streams that join it through
+ * events) between begin and end is recorded; launch replays it.  Only launches that allocate,
+ * set memory or synchronise nothing may be captured: the solvers' iteration launches once
+ * their workspaces exist. ---- */
+int tr_graph_begin(void* stream);
+int tr_graph_end(void* stream, tr_graph** out);
+int tr_graph_launch(tr_graph* graph, void* stream);
+int tr_graph_destroy(tr_graph* graph);
 
 /* ---- NUFFT plan (SigPy 0.1.27 nufft defaults: width 4, oversampling 1.25) ----
  * Replaces SigPyNufftBackend.__init__ (src/toporecon/nufft_backend.py:58-88):

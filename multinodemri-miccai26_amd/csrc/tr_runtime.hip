@@ -111,4 +111,46 @@ int tr_profile_read(int id, double* ms_total, int64_t* launches, double* bytes_t
   return kOk;
 }
 
+struct tr_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int tr_graph_begin(void* stream) {
+  TR_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return kOk;
+}
+
+int tr_graph_end(void* stream, tr_graph** out) {
+  if (!out) TR_FAIL(kInvalidArgument, "null output pointer");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  TR_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+  tr_graph* t = new tr_graph();
+  t->graph = g;
+  const hipError_t e = hipGraphInstantiate(&t->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    delete t;
+    TR_FAIL(kHipError, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  *out = t;
+  return kOk;
+}
+
+int tr_graph_launch(tr_graph* g, void* stream) {
+  if (!g) TR_FAIL(kInvalidArgument, "null graph");
+  TR_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return kOk;
+}
+
+int tr_graph_destroy(tr_graph* g) {
+  if (g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+  }
+  return kOk;
+}
+
 }  // extern "C"

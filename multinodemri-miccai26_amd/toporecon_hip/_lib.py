@@ -28,6 +28,10 @@ SIGNATURES = {
     "tr_profile_num_kernels": (c_int, []),
     "tr_profile_kernel_name": (ctypes.c_char_p, [c_int]),
     "tr_profile_read": (c_int, [c_int, c_dp, ctypes.POINTER(c_i64), c_dp]),
+    "tr_graph_begin": (c_int, [c_vp]),
+    "tr_graph_end": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "tr_graph_launch": (c_int, [c_vp, c_vp]),
+    "tr_graph_destroy": (c_int, [c_vp]),
     "tr_plan_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int,
                                ctypes.POINTER(c_vp), ctypes.POINTER(c_i64), c_vp]),
     "tr_plan_destroy": (c_int, [c_vp]),

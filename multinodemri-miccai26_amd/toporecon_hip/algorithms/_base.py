@@ -151,9 +151,20 @@ class HipPdhgBase:
         self._partial = torch.empty((n, 2), dtype=torch.float64, device=self.device)
         self._red = torch.empty((2,), dtype=torch.float64, device=self.device)
 
-    def _reduce(self):
+    def _reduce_launch(self):
         _lib.call("tr_reduce_partials", self._partial.data_ptr(), self._npartial, self._red.data_ptr(),
                   self.stream)
+
+    def _reduce(self):
+        self._reduce_launch()
+        num, den = self._red.tolist()  # device -> host, synchronises the stream
+        return num, den
+
+    def pdhg(self, it):
+        """One PDHG iteration; returns the float64 stopping sums (num, den).  The launches
+        (pdhg_launch) contain no host synchronisation, allocation or memset once the
+        workspaces exist, so one iteration can be captured into a HIP graph."""
+        self.pdhg_launch(it)
         num, den = self._red.tolist()  # device -> host, synchronises the stream
         return num, den
 

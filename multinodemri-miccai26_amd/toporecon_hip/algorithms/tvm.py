@@ -58,8 +58,9 @@ class TvmReconstructor(HipPdhgBase):
                   -1.0, self._partial.data_ptr(), s)
         return self._reduce()[0]
 
-    def pdhg(self, it):
-        """One PDHG iteration (tvm.py:227-380); returns (sum|du|^2, sum|u|^2) in float64."""
+    def pdhg_launch(self, it):
+        """Launch one PDHG iteration (tvm.py:227-380); the float64 stopping sums
+        (sum|du|^2, sum|u|^2) land in self._red."""
         dims = self.vol_dims()
         s = self.stream
         self.data_dual_update(self.ubar)
@@ -68,4 +69,4 @@ class TvmReconstructor(HipPdhgBase):
         self.data_adjoint(self.g)
         _lib.call("tr_tvm_primal", *dims, self.u.data_ptr(), self.ubar.data_ptr(), self.g.data_ptr(),
                   self.p_m.data_ptr(), float(self.tau), self._partial.data_ptr(), s)
-        return self._reduce()
+        self._reduce_launch()

@@ -72,8 +72,9 @@ class TvmeReconstructor(HipPdhgBase):
                   self._partial.data_ptr(), s)
         return self._reduce()[0]
 
-    def pdhg(self, it):
-        """One PDHG iteration (tvme.py:208-482); returns (sum|du|^2, sum|u|^2) in float64."""
+    def pdhg_launch(self, it):
+        """Launch one PDHG iteration (tvme.py:208-482); the float64 stopping sums
+        (sum|du|^2, sum|u|^2) land in self._red."""
         dims = self.vol_dims()
         s = self.stream
         self.data_dual_update(self.ubar)
@@ -84,4 +85,4 @@ class TvmeReconstructor(HipPdhgBase):
         _lib.call("tr_tvme_primal", *dims, self.u.data_ptr(), self.ubar.data_ptr(), self.g.data_ptr(),
                   self.p_m.data_ptr(), self.p_ex.data_ptr(), self.p_ey.data_ptr(), self.p_ez.data_ptr(),
                   float(self.tau), self._partial.data_ptr(), s)
-        return self._reduce()
+        self._reduce_launch()

@@ -122,8 +122,9 @@ class TvmwReconstructor(HipPdhgBase):
         # the TVMW stop sums compare ubar before (0) and after (2 u): num = 4 |u|^2
         return self._reduce()[0] / 4.0
 
-    def pdhg(self, it):
-        """One PDHG iteration (tvmw.py:385-685); returns (sum|dubar|^2, sum|ubar_prev|^2)."""
+    def pdhg_launch(self, it):
+        """Launch one PDHG iteration (tvmw.py:385-685); the float64 stopping sums
+        (sum|dubar|^2, sum|ubar_prev|^2) land in self._red."""
         dims = self.vol_dims()
         s = self.stream
         self.w2_forward(self.ubar, self.w2)
@@ -137,7 +138,7 @@ class TvmwReconstructor(HipPdhgBase):
         _lib.call("tr_tvmw_primal", *dims, self.u.data_ptr(), self.ubar.data_ptr(), self.g.data_ptr(),
                   self.p_m.data_ptr(), self.p_w1.data_ptr(), self.divw2.data_ptr(), float(self.tau),
                   self._partial.data_ptr(), s)
-        return self._reduce()
+        self._reduce_launch()
 
     def _converged_value(self, num, den):
         """TVMW rule: ||ubar_prev - ubar|| / max(||ubar_prev||, 1e-12), inf if not finite."""

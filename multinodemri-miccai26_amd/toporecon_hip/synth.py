@@ -86,6 +86,36 @@ def config_scaled(base: AcquisitionConfig, img_shape, coils=None, echoes=None, b
                              seed=base.seed if seed is None else seed)
 
 
+def batched_instance_configs(seeds=range(100, 112)):
+    """Independent TVM problems of uneven size from the small-TVM generator, sizes drawn per
+    seed: matrices from 32^3 (first seed) to 96x128x128 (last seed), 8-32 coils, 1-6 echoes,
+    4-8 bins, 0.2M-3M samples per echo (about 1.5-4 per voxel), tol 1e-3, at most 100
+    iterations."""
+    seeds = list(seeds)
+    out = []
+    for k, seed in enumerate(seeds):
+        rng = np.random.default_rng(seed)
+        if k == 0:
+            shape = (32, 32, 32)
+        elif k == len(seeds) - 1:
+            shape = (96, 128, 128)
+        else:
+            yx = int(rng.choice([32, 48, 64, 96, 128]))
+            shape = (int(rng.choice([32, 48, 64, 80, 96])), yx, yx)
+        coils = int(rng.choice([8, 12, 16, 24, 32]))
+        echoes = int(rng.integers(1, 7))
+        bins = int(rng.integers(4, 9))
+        nvox = shape[0] * shape[1] * shape[2]
+        samples = float(np.clip(nvox * rng.uniform(1.5, 4.0), 0.2e6, 3.0e6))
+        readouts = int(max(64, round(3 * max(shape) / 32) * 32))
+        interleaves = max(20 * bins, int(round(samples / readouts)))
+        cfg = config_scaled(config_small_tvm(), shape, coils=coils, echoes=echoes, bins=bins,
+                            interleaves=interleaves, readouts=readouts, seed=seed)
+        cfg.max_iter = 100
+        out.append(cfg)
+    return out
+
+
 def fibonacci_axes(n: int) -> np.ndarray:
     """n unit vectors (z, y, x) on a Fibonacci sphere."""
     i = np.arange(n, dtype=np.float64)
