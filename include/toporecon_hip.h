@@ -84,6 +84,9 @@ int tr_plan_release_workspace(tr_plan* plan);
 #define TR_PARAM_PRUNE 3
 #define TR_PARAM_SUPPORT_POINTS 4
 #define TR_PARAM_LIVE_COLUMNS 5
+/* TR_PARAM_INTERP_KGROUPS (default 0): 1 runs the interpolation one 64-channel group at a time
+ * (group-major), shrinking the grid footprint each XCD's L2 must hold. */
+#define TR_PARAM_INTERP_KGROUPS 6
 int tr_plan_set_param(tr_plan* plan, int param, int64_t value);
 int tr_plan_get_param(const tr_plan* plan, int param, int64_t* value);
 

@@ -27,6 +27,7 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
   const int ch = lane % KCI;
   if (m >= a.M) return;
   const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
+  const int kbase = blockIdx.y * KCI * NCI;  // channel group (gridDim.y > 1 in group-major mode)
   const TapRec& r = a.tap[m];
   const int packed = r.n;
   const int nx = (packed >> 8) & 15, ny = (packed >> 4) & 15;
@@ -40,7 +41,7 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
   // load sits behind a per-lane branch and a whole tap row is in flight at once
   int kk[NCI];
 #pragma unroll
-  for (int i = 0; i < NCI; ++i) kk[i] = min(i * KCI + ch, K - 1);
+  for (int i = 0; i < NCI; ++i) kk[i] = min(kbase + i * KCI + ch, K - 1);
   int px[5];
 #pragma unroll
   for (int dx = 0; dx < 5; ++dx) px[dx] = wrap2(p0x + dx, gx);
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
   }
 #pragma unroll
   for (int i = 0; i < NCI; ++i) {
-    const int k = i * KCI + ch;
+    const int k = kbase + i * KCI + ch;
     if (k >= K) continue;
     if (a.mode == kInterpPlain) {
       a.out[(int64_t)k * a.M + a.perm[m]] = acc[i];
@@ -143,16 +144,32 @@ __device__ __forceinline__ void cl_add(const ClusterPair& r, float2 v, int pl, f
   float2 tx[kClX];
 #pragma unroll
   for (int ix = 0; ix < kClX; ++ix) tx[ix] = make_float2(r.wx[ix] * v.x, r.wx[ix] * v.y);
+  if constexpr (NPL == 1) {
+    // one wave per (cluster, 64 channels): the pair and its row mask are wave-uniform, so the
+    // rows its taps miss are skipped with scalar branches
 #pragma unroll
-  for (int i = 0; i < ClLanes<KCI>::NP; ++i) {
-    const int pt = pl + i * NPL;
-    const int iz = pt / (kClY * kClX), iy = (pt / kClX) % kClY, ix = pt % kClX;
-    const float wzy = pick(r.wz, iz) * pick(r.wy, iy);
-    float2 t = tx[0];
+    for (int row = 0; row < kClZ * kClY; ++row) {
+      if (r.rows & (1 << row)) {
+        const float wzy = r.wz[row / kClY] * r.wy[row % kClY];
 #pragma unroll
-    for (int c = 1; c < kClX; ++c) t = ix == c ? tx[c] : t;
-    acc[i].x += wzy * t.x;
-    acc[i].y += wzy * t.y;
+        for (int ix = 0; ix < kClX; ++ix) {
+          acc[row * kClX + ix].x += wzy * tx[ix].x;
+          acc[row * kClX + ix].y += wzy * tx[ix].y;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ClLanes<KCI>::NP; ++i) {
+      const int pt = pl + i * NPL;
+      const int iz = pt / (kClY * kClX), iy = (pt / kClX) % kClY, ix = pt % kClX;
+      const float wzy = pick(r.wz, iz) * pick(r.wy, iy);
+      float2 t = tx[0];
+#pragma unroll
+      for (int c = 1; c < kClX; ++c) t = ix == c ? tx[c] : t;
+      acc[i].x += wzy * t.x;
+      acc[i].y += wzy * t.y;
+    }
   }
 }
 
@@ -307,6 +324,11 @@ int launch_interp(const InterpArgs& a, hipStream_t s) {
   const int spw = 64 / kci;
   const unsigned blocks = grid1d(a.M, 4 * spw);
   if (a.M == 0) return kOk;
+  if (a.kgroups && kci == 64) {  // one 64-channel group per grid row, groups outermost
+    hipLaunchKernelGGL((k_interp<64, 1>), dim3(blocks, nci), dim3(256), 0, s, a);
+    TR_LAUNCH_CHECK();
+    return kOk;
+  }
 #define TR_INTERP(KCI, NCI) hipLaunchKernelGGL((k_interp<KCI, NCI>), dim3(blocks), dim3(256), 0, s, a)
   switch (kci) {
     case 1: TR_INTERP(1, 1); break;

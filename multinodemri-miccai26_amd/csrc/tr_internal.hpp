@@ -47,7 +47,7 @@ struct __align__(16) ClusterPair {
   float wz[kClZ];
   float wy[kClY];
   float wx[kClX];  // includes 1 / width^3
-  int pad;
+  int rows;        // bit (z * kClY + y): the (z, y) point row has a nonzero weight
 };
 
 struct BinData {
@@ -91,6 +91,7 @@ struct tr_plan {
   // one XCD by the gather kernels (0 = hardware round-robin order)
   int interp_xcd_chunk = 64;
   int spread_xcd_chunk = 0;
+  int interp_kgroups = 0;  // 1: one 64-channel group per launch row (group-major order)
   // k-space support of all bins' taps: per (y, x) column of the oversampled
   // grid (natural order) the window (start, len) of z positions any tap
   // touches; len 0 = empty column.  Grid points outside it are never read by
@@ -172,6 +173,7 @@ enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2, kInte
 struct InterpArgs {
   GridGeom gg;
   int xcd_chunk;
+  int kgroups;
   const TapRec* tap;
   int64_t M;
   int K;

@@ -192,6 +192,7 @@ int tr_nufft_forward(tr_plan* p, int bin, const tr_c64* img, const tr_c64* maps,
   InterpArgs a = {};
   a.gg = geom(p);
   a.xcd_chunk = p->interp_xcd_chunk;
+  a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.M = b.M;
   a.K = E * C;
@@ -253,6 +254,7 @@ int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* m
   InterpArgs a = {};
   a.gg = geom(p);
   a.xcd_chunk = p->interp_xcd_chunk;
+  a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.M = b.M;
   a.K = E * C;
@@ -275,6 +277,7 @@ int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   InterpArgs a = {};
   a.gg = geom(p);
   a.xcd_chunk = p->interp_xcd_chunk;
+  a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.M = b.M;
   a.K = E * C;
@@ -294,6 +297,7 @@ int tr_data_residual(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   InterpArgs a = {};
   a.gg = geom(p);
   a.xcd_chunk = p->interp_xcd_chunk;
+  a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.M = b.M;
   a.K = E * C;

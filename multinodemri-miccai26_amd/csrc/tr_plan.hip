@@ -121,7 +121,10 @@ __global__ void k_fill_cluster_pairs(const TapRec* tap, int64_t M, const int* of
         for (int i = 0; i < kClZ; ++i) o.wz[i] = wz[a][i];
         for (int i = 0; i < kClY; ++i) o.wy[i] = wy[b][i];
         for (int i = 0; i < kClX; ++i) o.wx[i] = wx[c][i];
-        o.pad = 0;
+        o.rows = 0;
+        for (int i = 0; i < kClZ; ++i)
+          for (int k = 0; k < kClY; ++k)
+            if (o.wz[i] * o.wy[k] != 0.0f) o.rows |= 1 << (i * kClY + k);
         key[p] = (unsigned)(((int64_t)cz[a] * ncl.y + cy[b]) * ncl.z + cx[c]);
         idx[p] = p;
         rec[p] = o;
@@ -587,6 +590,10 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
       if (value != 0 && value != 1) TR_FAIL(kInvalidArgument, "prune is 0 or 1");
       p->prune = (int)value;
       return kOk;
+    case TR_PARAM_INTERP_KGROUPS:
+      if (value != 0 && value != 1) TR_FAIL(kInvalidArgument, "interp channel groups is 0 or 1");
+      p->interp_kgroups = (int)value;
+      return kOk;
     case TR_PARAM_SUPPORT_POINTS:
     case TR_PARAM_LIVE_COLUMNS:
       TR_FAIL(kInvalidArgument, "read-only plan parameter " + std::to_string(param));
@@ -600,6 +607,7 @@ int tr_plan_get_param(const tr_plan* p, int param, int64_t* value) {
     case TR_PARAM_INTERP_XCD_CHUNK: *value = p->interp_xcd_chunk; return kOk;
     case TR_PARAM_SPREAD_XCD_CHUNK: *value = p->spread_xcd_chunk; return kOk;
     case TR_PARAM_PRUNE: *value = p->prune; return kOk;
+    case TR_PARAM_INTERP_KGROUPS: *value = p->interp_kgroups; return kOk;
     case TR_PARAM_SUPPORT_POINTS: *value = p->support_points; return kOk;
     case TR_PARAM_LIVE_COLUMNS: *value = p->live_cols; return kOk;
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));

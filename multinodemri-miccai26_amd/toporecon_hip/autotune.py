@@ -21,6 +21,7 @@ import os
 from . import _lib
 
 INTERP_CANDIDATES = (0, 16, 64, 256, 1024)
+INTERP_KGROUPS = (0, 1)  # all channels per block / one 64-channel group per launch row
 SPREAD_CANDIDATES = (0, 64)
 
 _cache: dict = {}
@@ -64,6 +65,7 @@ def tune(plan, *, image, maps, E: int, C: int, ksp, dcf, stream, bins=(0,), reps
         hit = _cache.get(key) or disk.get(key)
         if hit:
             plan.set_param("interp_xcd_chunk", hit["interp_xcd_chunk"])
+            plan.set_param("interp_kgroups", hit.get("interp_kgroups", 0))
             plan.set_param("spread_xcd_chunk", hit["spread_xcd_chunk"])
             return dict(hit, cached=True)
     lib_call = _lib.call
@@ -80,17 +82,20 @@ def tune(plan, *, image, maps, E: int, C: int, ksp, dcf, stream, bins=(0,), reps
                      E, C, image.data_ptr(), 0, stream)
 
     fwd()  # warm the workspace at its full size
-    for c in INTERP_CANDIDATES:
-        plan.set_param("interp_xcd_chunk", c)
-        timings["interp"][c] = _time_kernel("interp", fwd, reps)
-    best_i = min(timings["interp"], key=timings["interp"].get)
+    for kg in INTERP_KGROUPS:
+        plan.set_param("interp_kgroups", kg)
+        for c in INTERP_CANDIDATES:
+            plan.set_param("interp_xcd_chunk", c)
+            timings["interp"][(kg, c)] = _time_kernel("interp", fwd, reps)
+    best_kg, best_i = min(timings["interp"], key=timings["interp"].get)
+    plan.set_param("interp_kgroups", best_kg)
     plan.set_param("interp_xcd_chunk", best_i)
     for c in SPREAD_CANDIDATES:
         plan.set_param("spread_xcd_chunk", c)
         timings["spread"][c] = _time_kernel("spread", adj, reps)
     best_s = min(timings["spread"], key=timings["spread"].get)
     plan.set_param("spread_xcd_chunk", best_s)
-    out = {"interp_xcd_chunk": best_i, "spread_xcd_chunk": best_s,
+    out = {"interp_xcd_chunk": best_i, "interp_kgroups": best_kg, "spread_xcd_chunk": best_s,
            "timings_ms": {k: {str(c): round(t, 4) for c, t in v.items()} for k, v in timings.items()}}
     _cache[key] = out
     if cache_path:

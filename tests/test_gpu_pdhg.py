@@ -90,9 +90,10 @@ def test_launch_params_and_autotune_keep_results_bitwise(torch_mod):
     acq, ksp, prob = make_problem(seed=11)
     base = _hip_solver("tvme", acq, ksp, 3, 0.0, True)
     ref = base.run().cpu().numpy()
-    for ci, cs in [(0, 0), (1, 1), (16, 64), (1024, 4096)]:
+    for ci, cs, kg in [(0, 0, 0), (1, 1, 1), (16, 64, 0), (1024, 4096, 1)]:
         base.plan.set_param("interp_xcd_chunk", ci)
         base.plan.set_param("spread_xcd_chunk", cs)
+        base.plan.set_param("interp_kgroups", kg)
         assert base.plan.get_param("interp_xcd_chunk") == ci
         out = base.run().cpu().numpy()
         assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (ci, cs)
