@@ -493,7 +493,7 @@ def main(argv=None):
                    "s_per_iteration": t_plain / max(n_it, 1),
                    "samples_per_bin": samples, "grid": list(solver.plan.grid_shape),
                    "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "interp_kgroups",
-                                                                "spread_xcd_chunk")},
+                                                                "spread_xcd_chunk", "bin_streams")},
                    "final_tol": solver.history[-1] if solver.history else None},
         "roofline": roofline,
         "cpu_baseline": cpu,

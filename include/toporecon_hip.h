@@ -66,7 +66,8 @@ int tr_graph_destroy(tr_graph* graph);
 int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
                    const int64_t* samples_per_bin, void* stream);
 int tr_plan_destroy(tr_plan* plan);
-/* info[0..2] image dims, [3..5] grid dims, [6] bins, [7] M_b, [8] nnz_b, [9] workspace channels. */
+/* info[0..2] image dims, [3..5] grid dims, [6] bins, [7] M_b, [8] nnz_b, [9] channels of the
+ * workspace last used. */
 int tr_plan_info(const tr_plan* plan, int bin, int64_t* info);
 int tr_plan_release_workspace(tr_plan* plan);
 
@@ -87,6 +88,10 @@ int tr_plan_release_workspace(tr_plan* plan);
 /* TR_PARAM_INTERP_KGROUPS (default 0): 1 runs the interpolation one 64-channel group at a time
  * (group-major), shrinking the grid footprint each XCD's L2 must hold. */
 #define TR_PARAM_INTERP_KGROUPS 6
+/* TR_PARAM_WORKSPACE_SLOTS (1..8, default 1): calls for bin b use grid workspace b % slots, so
+ * calls for bins in different slots may run concurrently on different streams (each slot holds
+ * its own oversampled grids). */
+#define TR_PARAM_WORKSPACE_SLOTS 7
 int tr_plan_set_param(tr_plan* plan, int param, int64_t value);
 int tr_plan_get_param(const tr_plan* plan, int param, int64_t* value);
 

@@ -69,6 +69,16 @@ struct BinData {
 
 }  // namespace tr
 
+namespace tr {
+constexpr int kMaxSlots = 8;
+struct Workspace {
+  float2* work = nullptr;
+  int64_t work_k = 0;
+  float2* partial = nullptr;
+  int64_t partial_n = 0;
+};
+}  // namespace tr
+
 struct tr_plan {
   int n[3];   // image z, y, x
   int g[3];   // oversampled grid
@@ -85,6 +95,12 @@ struct tr_plan {
   int64_t work_k = 0;
   float2* partial = nullptr;  // heavy-cluster segment partial sums
   int64_t partial_n = 0;
+  // work / work_k / partial / partial_n above belong to workspace slot `cur`; a call for
+  // bin b uses slot b % nslots, so calls for bins in different slots may run concurrently
+  // on different streams
+  tr::Workspace slots[tr::kMaxSlots];
+  int nslots = 1;
+  int cur = 0;
   int ncl[3];                 // clusters per axis
   int64_t NCL;
   // launch parameters (tr_plan_set_param): runs of consecutive blocks kept on
@@ -280,6 +296,7 @@ inline GridGeom geom(const tr_plan* p) {
 int ensure_partial(tr_plan* p, int64_t n);
 int ensure_work(tr_plan* p, int64_t K);
 int build_support(tr_plan* p, hipStream_t s);
+void use_slot(tr_plan* p, int slot);
 int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s);
 void free_plan(tr_plan* p);
 int check_plan_bin(const tr_plan* p, int bin);

@@ -184,6 +184,7 @@ extern "C" {
 int tr_nufft_forward(tr_plan* p, int bin, const tr_c64* img, const tr_c64* maps, int E, int C,
                      tr_c64* out, void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;
+  use_slot(p, bin % p->nslots);
   if (E < 1 || C < 1 || (!maps && C != 1)) TR_FAIL(kInvalidArgument, "need E >= 1 and C == 1 without maps");
   hipStream_t s = (hipStream_t)stream;
   const BinData& b = p->bins[bin];
@@ -206,6 +207,7 @@ int tr_nufft_forward(tr_plan* p, int bin, const tr_c64* img, const tr_c64* maps,
 int tr_nufft_adjoint(tr_plan* p, int bin, const tr_c64* ksp, const float* dcf, const tr_c64* maps, int E,
                      int C, tr_c64* out, int accumulate, void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;
+  use_slot(p, bin % p->nslots);
   if (E < 1 || C < 1 || (!maps && C != 1)) TR_FAIL(kInvalidArgument, "need E >= 1 and C == 1 without maps");
   hipStream_t s = (hipStream_t)stream;
   const BinData& b = p->bins[bin];
@@ -226,6 +228,7 @@ int tr_nufft_adjoint(tr_plan* p, int bin, const tr_c64* ksp, const float* dcf, c
 int tr_nufft_forward_grid(tr_plan* p, const tr_c64* img, const tr_c64* maps, int E, int C, tr_c64* grid,
                           void* stream) {
   if (!p) TR_FAIL(kInvalidArgument, "null plan");
+  use_slot(p, 0);
   if (E < 1 || C < 1 || (!maps && C != 1)) TR_FAIL(kInvalidArgument, "need E >= 1 and C == 1 without maps");
   hipStream_t s = (hipStream_t)stream;
   if (int r = forward_to_grid(p, F2(img), F2(maps), E, C, s, false)) return r;
@@ -236,6 +239,7 @@ int tr_nufft_forward_grid(tr_plan* p, const tr_c64* img, const tr_c64* maps, int
 int tr_nufft_adjoint_grid(tr_plan* p, const tr_c64* grid, const tr_c64* maps, int E, int C, tr_c64* out,
                           void* stream) {
   if (!p) TR_FAIL(kInvalidArgument, "null plan");
+  use_slot(p, 0);
   if (E < 1 || C < 1 || (!maps && C != 1)) TR_FAIL(kInvalidArgument, "need E >= 1 and C == 1 without maps");
   hipStream_t s = (hipStream_t)stream;
   if (int r = ensure_work(p, (int64_t)E * C)) return r;
@@ -247,6 +251,7 @@ int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* m
                         const tr_c64* y_sorted, tr_c64* q_sorted, float sigma, float one_plus_sigma,
                         void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;
+  use_slot(p, bin % p->nslots);
   hipStream_t s = (hipStream_t)stream;
   const BinData& b = p->bins[bin];
   if (b.M == 0) return kOk;
@@ -270,6 +275,7 @@ int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* m
 int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, int E, int C,
                      tr_c64* y_sorted, void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;
+  use_slot(p, bin % p->nslots);
   hipStream_t s = (hipStream_t)stream;
   const BinData& b = p->bins[bin];
   if (b.M == 0) return kOk;
@@ -290,6 +296,7 @@ int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
 int tr_data_residual(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, int E, int C,
                      const tr_c64* y_sorted, tr_c64* q_sorted, void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;
+  use_slot(p, bin % p->nslots);
   hipStream_t s = (hipStream_t)stream;
   const BinData& b = p->bins[bin];
   if (b.M == 0) return kOk;
@@ -311,6 +318,7 @@ int tr_data_residual(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
 int tr_data_adjoint(tr_plan* p, int bin, const tr_c64* q_sorted, const float* dcf_sorted,
                     const tr_c64* maps, int E, int C, tr_c64* g, int accumulate, void* stream) {
   if (int r = check_plan_bin(p, bin)) return r;
+  use_slot(p, bin % p->nslots);
   hipStream_t s = (hipStream_t)stream;
   const BinData& b = p->bins[bin];
   const int K = E * C;

@@ -198,6 +198,21 @@ __global__ void k_scatter_samples(const T* src, const int* perm, int64_t M, int 
   dst[(int64_t)k * M + perm[j]] = src[i];
 }
 
+void use_slot(tr_plan* p, int slot) {
+  if (slot == p->cur) return;
+  Workspace& c = p->slots[p->cur];
+  c.work = p->work;
+  c.work_k = p->work_k;
+  c.partial = p->partial;
+  c.partial_n = p->partial_n;
+  const Workspace& n = p->slots[slot];
+  p->work = n.work;
+  p->work_k = n.work_k;
+  p->partial = n.partial;
+  p->partial_n = n.partial_n;
+  p->cur = slot;
+}
+
 int ensure_partial(tr_plan* p, int64_t n) {
   if (p->partial_n >= n) return kOk;
   if (p->partial) {
@@ -447,8 +462,13 @@ void free_plan(tr_plan* p) {
     (void)hipFree(p->tw[i]);
     (void)hipFree(p->apod[i]);
   }
+  use_slot(p, 0);
   (void)hipFree(p->work);
   (void)hipFree(p->partial);
+  for (int i = 1; i < kMaxSlots; ++i) {
+    (void)hipFree(p->slots[i].work);
+    (void)hipFree(p->slots[i].partial);
+  }
   (void)hipFree(p->col_win);
   (void)hipFree(p->ywin);
   delete p;
@@ -569,12 +589,16 @@ int tr_plan_info(const tr_plan* p, int bin, int64_t* info) {
 
 int tr_plan_release_workspace(tr_plan* p) {
   if (!p) TR_FAIL(kInvalidArgument, "null plan");
-  if (p->work) {
-    TR_HIP(hipDeviceSynchronize());
-    TR_HIP(hipFree(p->work));
-  }
+  TR_HIP(hipDeviceSynchronize());
+  use_slot(p, 0);
+  TR_HIP(hipFree(p->work));
   p->work = nullptr;
   p->work_k = 0;
+  for (int i = 1; i < kMaxSlots; ++i) {
+    TR_HIP(hipFree(p->slots[i].work));
+    p->slots[i].work = nullptr;
+    p->slots[i].work_k = 0;
+  }
   return kOk;
 }
 
@@ -594,6 +618,10 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
       if (value != 0 && value != 1) TR_FAIL(kInvalidArgument, "interp channel groups is 0 or 1");
       p->interp_kgroups = (int)value;
       return kOk;
+    case TR_PARAM_WORKSPACE_SLOTS:
+      if (value < 1 || value > kMaxSlots) TR_FAIL(kInvalidArgument, "workspace slots outside [1, 8]");
+      p->nslots = (int)value;
+      return kOk;
     case TR_PARAM_SUPPORT_POINTS:
     case TR_PARAM_LIVE_COLUMNS:
       TR_FAIL(kInvalidArgument, "read-only plan parameter " + std::to_string(param));
@@ -608,6 +636,7 @@ int tr_plan_get_param(const tr_plan* p, int param, int64_t* value) {
     case TR_PARAM_SPREAD_XCD_CHUNK: *value = p->spread_xcd_chunk; return kOk;
     case TR_PARAM_PRUNE: *value = p->prune; return kOk;
     case TR_PARAM_INTERP_KGROUPS: *value = p->interp_kgroups; return kOk;
+    case TR_PARAM_WORKSPACE_SLOTS: *value = p->nslots; return kOk;
     case TR_PARAM_SUPPORT_POINTS: *value = p->support_points; return kOk;
     case TR_PARAM_LIVE_COLUMNS: *value = p->live_cols; return kOk;
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));

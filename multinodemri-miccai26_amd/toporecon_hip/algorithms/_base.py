@@ -177,21 +177,53 @@ class HipPdhgBase:
             else:
                 self.q[b].copy_(self.q0[b])
 
+    def set_bin_streams(self, n: int):
+        """Issue the data term of bin b on stream b % n with its own grid workspace (n = 1: one
+        stream).  Results are bit-identical for every n; n > 1 lets bins overlap on the GPU."""
+        import torch
+        n = max(1, min(int(n), 8, max(self.B, 1)))
+        self.plan.set_param("workspace_slots", n)
+        self._bstreams = [torch.cuda.Stream(self.device) for _ in range(n)] if n > 1 else None
+
+    def _bin_streams(self):
+        """Per-bin stream handles, forked from the current stream (None: use the current one)."""
+        import torch
+        bs = getattr(self, "_bstreams", None)
+        if not bs:
+            return None
+        main = torch.cuda.current_stream(self.device)
+        for st in bs:
+            st.wait_stream(main)
+        return [bs[b % len(bs)].cuda_stream for b in range(self.B)]
+
+    def _join_bin_streams(self):
+        import torch
+        bs = getattr(self, "_bstreams", None)
+        if bs:
+            main = torch.cuda.current_stream(self.device)
+            for st in bs:
+                main.wait_stream(st)
+
     def data_dual_update(self, ubar):
         """q_b <- (q_b + sigma (A ubar_b - y_b)) / (1 + sigma) for every bin (one launch chain per bin)."""
         s = self.stream
+        hs = self._bin_streams()
         sig = float(self.sigma)
         ops = float(np.float32(1.0 + self.sigma))
         for b in range(self.B):
             _lib.call("tr_data_dual_update", self.plan.handle, b, ubar[b].data_ptr(), self.maps.data_ptr(),
-                      self.E, self.C, self.y[b].data_ptr(), self.q[b].data_ptr(), sig, ops, s)
+                      self.E, self.C, self.y[b].data_ptr(), self.q[b].data_ptr(), sig, ops,
+                      hs[b] if hs else s)
+        self._join_bin_streams()
 
     def data_adjoint(self, g):
         """g_b = sum_c conj(S_c) NUFFT_b^H(dcf_b q_b,c) for every bin."""
         s = self.stream
+        hs = self._bin_streams()
         for b in range(self.B):
             _lib.call("tr_data_adjoint", self.plan.handle, b, self.q[b].data_ptr(), self.dcf[b].data_ptr(),
-                      self.maps.data_ptr(), self.E, self.C, g[b].data_ptr(), 0, s)
+                      self.maps.data_ptr(), self.E, self.C, g[b].data_ptr(), 0, hs[b] if hs else s)
+        self._join_bin_streams()
 
     def simulate_kspace(self, truth, snr_db=30.0, seed=0, normalize=True):
         """Replace y by A truth + complex Gaussian noise at snr_db, then y /= max|y|."""
@@ -219,12 +251,17 @@ class HipPdhgBase:
                 self.y[b].div_(np.float32(mx))
         return mx
 
-    def autotune(self, reps: int = 2, cache_path=None, force: bool = False) -> dict:
-        """Pick the plan's gridding launch parameters by timing them on the largest bin.
+    def autotune(self, reps: int = 2, cache_path=None, force: bool = False, bin_streams: bool = True,
+                 workspace_budget: float = 64 * 2.0 ** 30) -> dict:
+        """Pick the plan's gridding launch parameters by timing them on the largest bin, then the
+        number of bin streams (each with its own grid workspace, within workspace_budget bytes)
+        by timing the adjoint data term.
 
-        Only the workgroup order changes, so run() gives bit-identical results
-        whatever is chosen.
+        Only the workgroup order and stream assignment change, so run() gives bit-identical
+        results whatever is chosen.
         """
+        import time as _time
+
         import torch
         from .. import autotune as _autotune
         if self.B == 0:
@@ -238,6 +275,27 @@ class HipPdhgBase:
                              dcf=self.dcf[b], stream=self.stream, bins=(b,), reps=reps,
                              cache_path=cache_path, force=force)
         del img, ksp
+        if bin_streams and self.B > 1:
+            ws = float(np.prod(self.plan.grid_shape)) * self.K * 8.0
+            cands = sorted({n for n in (1, 2, 3, self.B) if 1 <= n <= min(self.B, 8) and n * ws <= workspace_budget})
+            g = torch.empty(self.vol_dims(), dtype=torch.complex64, device=self.device)
+            times = {}
+            for n in cands:
+                self.set_bin_streams(n)
+                self.data_adjoint(g)
+                torch.cuda.synchronize(self.device)
+                t0 = _time.perf_counter()
+                for _ in range(max(1, reps)):
+                    self.data_adjoint(g)
+                torch.cuda.synchronize(self.device)
+                times[n] = (_time.perf_counter() - t0) / max(1, reps) * 1e3
+            best = min(times, key=times.get)
+            self.set_bin_streams(best)
+            if best < max(cands):
+                self.plan.release_workspace()  # slots beyond `best` are no longer used
+            del g
+            out = dict(out, bin_streams=best, timings_ms=dict(out.get("timings_ms", {}),
+                                                              bin_streams={str(k): round(v, 3) for k, v in times.items()}))
         return out
 
     # -- step-size check ------------------------------------------------------------

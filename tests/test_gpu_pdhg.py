@@ -97,6 +97,10 @@ def test_launch_params_and_autotune_keep_results_bitwise(torch_mod):
         assert base.plan.get_param("interp_xcd_chunk") == ci
         out = base.run().cpu().numpy()
         assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (ci, cs)
+    for n in (3, 2, 1):  # bins on several streams, one grid workspace per stream
+        base.set_bin_streams(n)
+        out = base.run().cpu().numpy()
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), n
     tuned = base.autotune(reps=1, force=True)
     assert tuned["interp_xcd_chunk"] in (0, 16, 64, 256, 1024)
     out = base.run().cpu().numpy()
