@@ -60,6 +60,11 @@ def parse_args(argv=None):
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarize two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, "
                          "WRITE_SIZE) of a run with --copy-calibration-mib into profiles/traffic.json")
+    ap.add_argument("--params", default=None,
+                    help="JSON launch parameters (interp_xcd_chunk, interp_kgroups, spread_xcd_chunk, "
+                         "bin_streams) to set instead of autotuning (profiling runs)")
+    ap.add_argument("--legs", choices=("all", "plain"), default="all",
+                    help="plain: only the plain PDHG leg (no warm start, no power iteration)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the solve, copy a buffer of this many MiB once (a kernel of known "
                          "traffic for scaling hardware byte counters)")
@@ -362,13 +367,23 @@ def main(argv=None):
         log(f"copy calibration: one copy of {4 * n} bytes (read {4 * n}, write {4 * n})", verbose)
         del src, dst
     t0 = time.perf_counter()
-    tuned = solver.autotune()
-    log(f"autotune in {time.perf_counter() - t0:.1f} s: {tuned}", verbose)
+    if args.params:
+        tuned = json.loads(args.params)
+        for k in ("interp_xcd_chunk", "interp_kgroups", "spread_xcd_chunk"):
+            if k in tuned:
+                solver.plan.set_param(k, int(tuned[k]))
+        solver.set_bin_streams(int(tuned.get("bin_streams", 1)))
+        log(f"launch parameters set: {tuned}", verbose)
+    else:
+        tuned = solver.autotune()
+        log(f"autotune in {time.perf_counter() - t0:.1f} s: {tuned}", verbose)
 
     from toporecon_hip import objective
     from toporecon_hip.learned import WarmStartNet, cnn_warm_start, default_weights_path
     net = None
-    if os.path.exists(default_weights_path()):
+    if args.legs == "plain":
+        log("plain leg only", verbose)
+    elif os.path.exists(default_weights_path()):
         net = WarmStartNet.load(default_weights_path(), dev)
     else:
         log("no trained warm-start weights: the learned leg is skipped (plain PDHG only)", verbose)
@@ -384,8 +399,7 @@ def main(argv=None):
         log(f"warmup {w} done", verbose)
     solver.max_iter = full_iter
 
-    # plain PDHG from zeros (kernel profile from this leg)
-    _lib.profile_enable(True)
+    # plain PDHG from zeros
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     iters = []
@@ -394,8 +408,6 @@ def main(argv=None):
         iters.append(solver.iterations)
     torch.cuda.synchronize(dev)
     elapsed = time.perf_counter() - t0
-    prof = _lib.profile_read()
-    _lib.profile_enable(False)
     t_plain = elapsed / args.steps
     n_it = iters[-1]
     u_plain = u_plain.clone()
@@ -427,13 +439,30 @@ def main(argv=None):
         log(f"warm start: {solver.iterations} iterations, {t_warm:.2f} s (start {t_start:.2f} s), objective "
             f"{phi_warm:.6e} ({rel:+.2e} vs plain), speedup {t_plain / t_warm:.2f}x", verbose)
     # step-size check: float64 power iteration for ||K|| (outside the timed legs)
-    t0 = time.perf_counter()
-    knorm = solver.operator_norm(iters=20)
-    knorm = {k: knorm[k] for k in ("K_norm", "sigma_tau_K_norm_sq", "iterations")}
-    log(f"power iteration: ||K|| = {knorm['K_norm']:.4f}, sigma tau ||K||^2 = "
-        f"{knorm['sigma_tau_K_norm_sq']:.4f} ({time.perf_counter() - t0:.1f} s)", verbose)
+    knorm = None
+    if args.legs == "all":
+        t0 = time.perf_counter()
+        knorm = solver.operator_norm(iters=20)
+        knorm = {k: knorm[k] for k in ("K_norm", "sigma_tau_K_norm_sq", "iterations")}
+        log(f"power iteration: ||K|| = {knorm['K_norm']:.4f}, sigma tau ||K||^2 = "
+            f"{knorm['sigma_tau_K_norm_sq']:.4f} ({time.perf_counter() - t0:.1f} s)", verbose)
     use_warm = warm is not None and warm["within_0.1pct"]
     t_step = warm["time_to_tol_s"] if use_warm else t_plain
+
+    final_tol = solver.history[-1] if solver.history else None
+    # per-kernel times: a short serial pass (bins on one stream), so that no two kernels overlap
+    # and each HIP-event interval is one kernel's own duration
+    streams_used = len(getattr(solver, "_bstreams", None) or [None])
+    solver.set_bin_streams(1)
+    solver.max_iter, prof_iters = min(3, full_iter), min(3, full_iter)
+    solver.tol = 0.0
+    _lib.profile_enable(True)
+    solver.run()
+    torch.cuda.synchronize(dev)
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    solver.max_iter, solver.tol = full_iter, cfg.tol
+    solver.set_bin_streams(streams_used)
 
     kernels = {}
     for name, (ms, n, by) in prof.items():
@@ -455,7 +484,7 @@ def main(argv=None):
                  if k in kernels)
     adj_ms = sum(kernels[k]["ms_total"] for k in ("spread", "fft_z_inv", "fft_y_inv", "adj_x_crop")
                  if k in kernels)
-    total_iters = sum(iters)
+    total_iters = prof_iters
     nufft = {"forward_gbps": fwd_b * total_iters / (fwd_ms * 1e-3) / 1e9 if fwd_ms else None,
              "adjoint_gbps": adj_b * total_iters / (adj_ms * 1e-3) / 1e9 if adj_ms else None,
              "forward_ms_per_iter": fwd_ms / total_iters, "adjoint_ms_per_iter": adj_ms / total_iters}
@@ -494,7 +523,8 @@ def main(argv=None):
                    "samples_per_bin": samples, "grid": list(solver.plan.grid_shape),
                    "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "interp_kgroups",
                                                                 "spread_xcd_chunk", "bin_streams")},
-                   "final_tol": solver.history[-1] if solver.history else None},
+                   "final_tol": final_tol, "kernel_profile": f"serial pass, {prof_iters} iterations, "
+                                                             "bins on one stream"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "nufft": nufft,
