@@ -45,26 +45,46 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
   int px[5];
 #pragma unroll
   for (int dx = 0; dx < 5; ++dx) px[dx] = wrap2(p0x + dx, gx);
+  // One z plane at a time: the 4 x 4 taps of its first four y rows (every sample has them) are
+  // loaded together before any is used, so 16 * NCI loads are in flight per wave; the rare fifth
+  // y row / x tap (a coordinate on a grid line) follow.  Products are added in (dz, dy, dx) order.
   for (int dz = 0; dz < nz; ++dz) {
     const int pz = wrap2(p0z + dz, gz);
     const float wz = r.w[dz];
-    for (int dy = 0; dy < ny; ++dy) {
-      const int py = wrap2(p0y + dy, gy);
-      const float wzy = wz * r.w[5 + dy];
-      const float2* row = a.grid + (((int64_t)pz * gy + py) * gx) * K;
-      // four x taps always exist; a fifth only for a coordinate on a grid line
-      float2 v[4][NCI];
+    float2 v[4][4][NCI];
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+      const float2* row = a.grid + (((int64_t)pz * gy + wrap2(p0y + dy, gy)) * gx) * K;
 #pragma unroll
       for (int dx = 0; dx < 4; ++dx)
 #pragma unroll
-        for (int i = 0; i < NCI; ++i) v[dx][i] = row[(int64_t)px[dx] * K + kk[i]];
+        for (int i = 0; i < NCI; ++i) v[dy][dx][i] = row[(int64_t)px[dx] * K + kk[i]];
+    }
 #pragma unroll
-      for (int dx = 0; dx < 4; ++dx) {
-        const float w = wzy * r.w[10 + dx];
+    for (int dy = 0; dy < 5; ++dy) {
+      if (dy == 4 && ny < 5) break;
+      const float wzy = wz * r.w[5 + dy];
+      const float2* row = a.grid + (((int64_t)pz * gy + wrap2(p0y + dy, gy)) * gx) * K;
+      if (dy < 4) {
 #pragma unroll
-        for (int i = 0; i < NCI; ++i) {
-          acc[i].x += w * v[dx][i].x;
-          acc[i].y += w * v[dx][i].y;
+        for (int dx = 0; dx < 4; ++dx) {
+          const float w = wzy * r.w[10 + dx];
+#pragma unroll
+          for (int i = 0; i < NCI; ++i) {
+            acc[i].x += w * v[dy][dx][i].x;
+            acc[i].y += w * v[dy][dx][i].y;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int dx = 0; dx < 4; ++dx) {
+          const float w = wzy * r.w[10 + dx];
+#pragma unroll
+          for (int i = 0; i < NCI; ++i) {
+            const float2 u = row[(int64_t)px[dx] * K + kk[i]];
+            acc[i].x += w * u.x;
+            acc[i].y += w * u.y;
+          }
         }
       }
       if (nx == 5) {
@@ -173,21 +193,22 @@ __device__ __forceinline__ void cl_add(const ClusterPair& r, float2 v, int pl, f
   }
 }
 
-// acc += sum over pairs [j0, j1) in increasing order (four pairs' loads are
-// issued together; the additions stay in order)
+// acc += sum over pairs [j0, j1) in increasing order (the sample loads of
+// kSpreadUnroll pairs are issued together; the additions stay in order)
 template <int KCI, int SORTED, int DCF>
 __device__ __forceinline__ void cl_run_impl(const SpreadArgs& a, int j0, int j1, int kk, int pl,
                                             float2 (&acc)[ClLanes<KCI>::NP]) {
   int j = j0;
-  for (; j + 4 <= j1; j += 4) {
-    ClusterPair r[4];
-    float2 v[4];
+  constexpr int U = kSpreadUnroll;
+  for (; j + U <= j1; j += U) {
+    ClusterPair r[U];
+    float2 v[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) r[u] = a.pairs[j + u];
+    for (int u = 0; u < U; ++u) r[u] = a.pairs[j + u];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = spread_src<SORTED, DCF>(a, r[u].m, kk);
+    for (int u = 0; u < U; ++u) v[u] = spread_src<SORTED, DCF>(a, r[u].m, kk);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) cl_add<KCI>(r[u], v[u], pl, acc);
+    for (int u = 0; u < U; ++u) cl_add<KCI>(r[u], v[u], pl, acc);
   }
   for (; j < j1; ++j) {
     const ClusterPair r = a.pairs[j];

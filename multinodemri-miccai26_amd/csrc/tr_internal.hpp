@@ -24,6 +24,10 @@ constexpr int kWidth = 4;
 constexpr int kTile = 8;  // sample sort tile edge (grid points)
 constexpr int kMaxKc = 16;  // channels per FFT workgroup
 constexpr int kSpreadSeg = 256;  // samples per heavy-cluster segment
+#ifndef TR_SPREAD_UNROLL
+#define TR_SPREAD_UNROLL 4
+#endif
+constexpr int kSpreadUnroll = TR_SPREAD_UNROLL;  // pairs whose sample loads are in flight together
 
 // Per-sample taps (sorted order): natural positions of the first tap per axis,
 // tap counts (4 or 5) and float32 Kaiser-Bessel weights (x weights already
