@@ -10,10 +10,13 @@ full solve; inputs are resident in HBM when the timed region starts.
 
 Two legs run on the same problem: plain PDHG from zeros (as the reference),
 and the learned warm start (density-compensated adjoint -> 3-D CNN on MFMA ->
-per-(bin, echo) data-fitted scale -> PDHG from there to the same tolerance,
-the warm-start work inside the timed region).  ``value`` is the learned leg's
-time-to-tol when its final objective is within 0.1 % of the plain leg's,
-otherwise the plain leg's (the fallback); both legs are reported.
+per-(bin, echo) data-fitted scale -> 3 conjugate-gradient data-consistency
+steps -> PDHG from there to the same tolerance, the warm-start work inside the
+timed region).  ``value`` is the learned leg's time-to-tol when its final
+objective is within +-0.1 % of the plain leg's, otherwise the plain leg's (the
+fallback); both legs are reported.  Variant starts (CNN without CG, CG from
+zeros without the CNN, CNN with 5 CG steps) are timed too and reported with
+their signed objective differences; they never set ``value``.
 
 Prints one JSON line: time-to-tol (value, seconds), iterations, per-kernel
 algorithmic GB/s from HIP events recorded on the launch stream (plain leg),
@@ -65,6 +68,11 @@ def parse_args(argv=None):
                          "bin_streams) to set instead of autotuning (profiling runs)")
     ap.add_argument("--legs", choices=("all", "plain"), default="all",
                     help="plain: only the plain PDHG leg (no warm start, no power iteration)")
+    ap.add_argument("--cg-steps", type=int, default=None,
+                    help="CG steps of the learned leg (default: the warm-start module's default, 3)")
+    ap.add_argument("--variants", default="cnn+0,zero+3,cnn+5",
+                    help="comma list of extra warm starts 'cnn+K' / 'zero+K' (K CG steps) timed and "
+                         "reported beside the learned leg; '' for none")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the solve, copy a buffer of this many MiB once (a kernel of known "
                          "traffic for scaling hardware byte counters)")
@@ -400,7 +408,8 @@ def main(argv=None):
         log(f"autotune in {time.perf_counter() - t0:.1f} s: {tuned}", verbose)
 
     from toporecon_hip import objective
-    from toporecon_hip.learned import WarmStartNet, cnn_warm_start, default_weights_path
+    from toporecon_hip.learned import DEFAULT_CG_STEPS, WarmStartNet, default_weights_path, warm_start
+    cg_steps = DEFAULT_CG_STEPS if args.cg_steps is None else args.cg_steps
     net = None
     if args.legs == "plain":
         log("plain leg only", verbose)
@@ -415,7 +424,7 @@ def main(argv=None):
     for w in range(args.warmup):
         solver.run()
         if net is not None:
-            solver.run(init=cnn_warm_start(solver, net), dual_init="residual")
+            solver.run(init=warm_start(solver, net, cg_steps=cg_steps), dual_init="residual")
         torch.cuda.synchronize(dev)
         log(f"warmup {w} done", verbose)
     solver.max_iter = full_iter
@@ -437,28 +446,41 @@ def main(argv=None):
     log(f"plain: {args.steps} solve(s), {n_it} iterations, {t_plain:.2f} s per solve, objective {phi_plain:.6e}",
         verbose)
 
-    # learned warm start: dcf adjoint -> CNN (MFMA, bf16) -> data-fitted scale -> PDHG to the same tol
-    warm = None
-    if net is not None:
+    # learned warm start: dcf adjoint -> CNN (MFMA, bf16) -> data-fitted scale -> CG steps -> PDHG
+    def timed_start(use_net, k_cg, steps):
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        for k in range(args.steps):
+        for _ in range(steps):
             ta = time.perf_counter()
-            u0 = cnn_warm_start(solver, net)
+            u0 = warm_start(solver, net if use_net else None, cg_steps=k_cg)
             torch.cuda.synchronize(dev)
             t_start = time.perf_counter() - ta
             u_warm = solver.run(init=u0, dual_init="residual", progress_every=25)
         torch.cuda.synchronize(dev)
-        t_warm = (time.perf_counter() - t0) / args.steps
+        t_warm = (time.perf_counter() - t0) / steps
         phi_warm = objective.objective(solver, u_warm)
         rel = (phi_warm - phi_plain) / phi_plain
-        warm = {"time_to_tol_s": t_warm, "iterations": solver.iterations, "warm_start_s": t_start,
-                "objective": phi_warm, "objective_plain": phi_plain, "objective_rel_diff": rel,
-                "within_0.1pct": abs(rel) <= 1e-3, "speedup_vs_plain": t_plain / t_warm,
-                "weights": os.path.basename(default_weights_path()), "trained_on": net.meta}
         del u0, u_warm
-        log(f"warm start: {solver.iterations} iterations, {t_warm:.2f} s (start {t_start:.2f} s), objective "
-            f"{phi_warm:.6e} ({rel:+.2e} vs plain), speedup {t_plain / t_warm:.2f}x", verbose)
+        return {"start": ("cnn" if use_net else "zero") + f"+{k_cg}cg", "cg_steps": k_cg,
+                "time_to_tol_s": t_warm, "iterations": solver.iterations, "warm_start_s": t_start,
+                "objective": phi_warm, "objective_plain": phi_plain, "objective_rel_diff": rel,
+                "within_0.1pct": abs(rel) <= 1e-3, "not_worse_than_plain_0.1pct": rel <= 1e-3,
+                "speedup_vs_plain": t_plain / t_warm}
+
+    warm = None
+    variants = []
+    if net is not None:
+        warm = timed_start(True, cg_steps, args.steps)
+        warm.update(weights=os.path.basename(default_weights_path()), trained_on=net.meta)
+        log(f"warm start {warm['start']}: {warm['iterations']} iterations, {warm['time_to_tol_s']:.2f} s "
+            f"(start {warm['warm_start_s']:.2f} s), objective {warm['objective']:.6e} "
+            f"({warm['objective_rel_diff']:+.2e} vs plain), speedup {warm['speedup_vs_plain']:.2f}x", verbose)
+        for v in filter(None, (x.strip() for x in args.variants.split(","))):
+            kind, k_cg = v.split("+")
+            r = timed_start(kind == "cnn", int(k_cg), 1)
+            variants.append(r)
+            log(f"variant {r['start']}: {r['iterations']} iterations, {r['time_to_tol_s']:.2f} s, "
+                f"objective {r['objective_rel_diff']:+.2e} vs plain, speedup {r['speedup_vs_plain']:.2f}x", verbose)
     # step-size check: float64 power iteration for ||K|| (outside the timed legs)
     knorm = None
     if args.legs == "all":
@@ -530,7 +552,7 @@ def main(argv=None):
         "data": "synthetic: seeded cone trajectory, ellipsoid phantom, Gaussian-blob coil maps, "
                 "k-space simulated by the HIP forward NUFFT + 30 dB complex noise",
         "config": {"workload": f"TVME PDHG to tol {cfg.tol} (max {cfg.max_iter} it) "
-                               f"{'from the learned CNN warm start' if use_warm else 'from zeros (plain)'}, "
+                               f"{f'from the learned CNN + {cg_steps} CG warm start' if use_warm else 'from zeros (plain)'}, "
                                f"{shape}, {cfg.coils} coils, {cfg.echoes} echoes, {cfg.bins} bins, "
                                f"{cfg.interleaves}x{cfg.readouts} cone samples/echo, "
                                f"l2 coupling, lambda {cfg.lambda_motion}",
@@ -538,6 +560,7 @@ def main(argv=None):
                    "plain": {"time_to_tol_s": t_plain, "iterations": n_it, "objective": phi_plain,
                              "s_per_iteration": t_plain / max(n_it, 1)},
                    "learned_warm_start": warm,
+                   "warm_start_variants": variants,
                    "operator_norm": knorm,
                    "iterations": warm["iterations"] if use_warm else n_it,
                    "s_per_iteration": t_plain / max(n_it, 1),

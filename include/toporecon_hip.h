@@ -144,6 +144,14 @@ int tr_data_residual(tr_plan* plan, int bin, const tr_c64* x, const tr_c64* maps
 int tr_data_adjoint(tr_plan* plan, int bin, const tr_c64* q_sorted, const float* dcf_sorted,
                     const tr_c64* maps, int E, int C, tr_c64* g_b, int accumulate, void* stream);
 
+/* Conjugate-gradient vector steps (the warm start's data-consistency refinement; no reference
+ * counterpart).  partial[b] receives block b's float64 sum (nblocks blocks, added in order by the
+ * caller): update x += alpha p, r -= alpha Ap, sum |r|^2; direction p = r + beta p;
+ * dot sum Re(conj(a) b). */
+int tr_cg_update(tr_c64* x, tr_c64* r, const tr_c64* p, const tr_c64* ap, int64_t n, double alpha,
+                 double* partial, int nblocks, void* stream);
+int tr_cg_direction(tr_c64* p, const tr_c64* r, int64_t n, double beta, void* stream);
+int tr_cdot_re(const tr_c64* a, const tr_c64* b, int64_t n, double* partial, int nblocks, void* stream);
 /* out = s * x (complex64, n values): the power iteration's normalisation. */
 int tr_cscale(const tr_c64* x, int64_t n, double s, tr_c64* out, void* stream);
 /* ---- element-wise PDHG steps on (B, E, Z, Y, X) arrays ----

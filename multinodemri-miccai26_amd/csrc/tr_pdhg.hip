@@ -301,6 +301,52 @@ __global__ void __launch_bounds__(kEwBlock) k_cscale(const float2* x, int64_t n,
   if (i < n) out[i] = make_float2(s * x[i].x, s * x[i].y);
 }
 
+// Conjugate-gradient vector steps for the data-consistency refinement of a warm
+// start: x += alpha p, r -= alpha Ap with per-block float64 sums of |r|^2; p = r +
+// beta p; per-block float64 sums of Re(conj(a) b).  Block sums are added in block
+// order on the host, so every result is reproducible.
+__device__ __forceinline__ void block_sum1(double v, double* partial) {
+  __shared__ double sh[kEwBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kEwBlock / 64; ++w) t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kEwBlock) k_cg_update(float2* x, float2* r, const float2* p, const float2* ap,
+                                                        int64_t n, float alpha, double* partial) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 pv = p[i], av = ap[i];
+    float2 xv = x[i], rv = r[i];
+    xv = make_float2(xv.x + alpha * pv.x, xv.y + alpha * pv.y);
+    rv = make_float2(rv.x - alpha * av.x, rv.y - alpha * av.y);
+    x[i] = xv;
+    r[i] = rv;
+    acc += (double)rv.x * rv.x + (double)rv.y * rv.y;
+  }
+  block_sum1(acc, partial);
+}
+
+__global__ void __launch_bounds__(kEwBlock) k_cg_direction(float2* p, const float2* r, int64_t n, float beta) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float2 pv = p[i], rv = r[i];
+    p[i] = make_float2(rv.x + beta * pv.x, rv.y + beta * pv.y);
+  }
+}
+
+__global__ void __launch_bounds__(kEwBlock) k_cdot_re(const float2* a, const float2* b, int64_t n, double* partial) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)a[i].x * b[i].x + (double)a[i].y * b[i].y;
+  block_sum1(acc, partial);
+}
+
 __global__ void __launch_bounds__(kEwBlock) k_reduce_partials(const double2* partial, int n, double* out) {
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < n; i += kEwBlock) {
@@ -414,6 +460,34 @@ int tr_tvmw_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, c
   ProfScope ps(kKTvmwPrimal, 9.0 * 8.0 * g.B * g.E * g.N, (hipStream_t)stream);
   hipLaunchKernelGGL(k_tvmw_primal, dim3(ew_blocks(g)), dim3(kEwBlock), 0, (hipStream_t)stream, g, F2(u),
                      F2(ubar), F2(grad), F2(p_m), F2(p_w1), F2(divw2), tau, (double2*)partial);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_cg_update(tr_c64* x, tr_c64* r, const tr_c64* p, const tr_c64* ap, int64_t n, double alpha,
+                 double* partial, int nblocks, void* stream) {
+  if (!x || !r || !p || !ap || !partial) TR_FAIL(kInvalidArgument, "null argument");
+  if (nblocks < 1 || nblocks > 65536) TR_FAIL(kInvalidArgument, "nblocks outside [1, 65536]");
+  hipLaunchKernelGGL(k_cg_update, dim3(nblocks), dim3(kEwBlock), 0, (hipStream_t)stream, (float2*)x, (float2*)r,
+                     (const float2*)p, (const float2*)ap, n, (float)alpha, partial);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_cg_direction(tr_c64* p, const tr_c64* r, int64_t n, double beta, void* stream) {
+  if ((!p || !r) && n > 0) TR_FAIL(kInvalidArgument, "null argument");
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_cg_direction, dim3((unsigned)((n + kEwBlock - 1) / kEwBlock)), dim3(kEwBlock), 0,
+                     (hipStream_t)stream, (float2*)p, (const float2*)r, n, (float)beta);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_cdot_re(const tr_c64* a, const tr_c64* b, int64_t n, double* partial, int nblocks, void* stream) {
+  if (!a || !b || !partial) TR_FAIL(kInvalidArgument, "null argument");
+  if (nblocks < 1 || nblocks > 65536) TR_FAIL(kInvalidArgument, "nblocks outside [1, 65536]");
+  hipLaunchKernelGGL(k_cdot_re, dim3(nblocks), dim3(kEwBlock), 0, (hipStream_t)stream, (const float2*)a,
+                     (const float2*)b, n, partial);
   TR_LAUNCH_CHECK();
   return kOk;
 }

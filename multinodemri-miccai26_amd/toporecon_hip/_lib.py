@@ -52,6 +52,9 @@ SIGNATURES = {
     "tr_data_residual": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
     "tr_data_adjoint": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_int, c_vp]),
     "tr_partial_count": (c_i64, [c_int, c_int, c_int, c_int]),
+    "tr_cg_update": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_double, c_vp, c_int, c_vp]),
+    "tr_cg_direction": (c_int, [c_vp, c_vp, c_i64, ctypes.c_double, c_vp]),
+    "tr_cdot_re": (c_int, [c_vp, c_vp, c_i64, c_vp, c_int, c_vp]),
     "tr_cscale": (c_int, [c_vp, c_i64, ctypes.c_double, c_vp, c_vp]),
     "tr_reduce_partials": (c_int, [c_vp, c_i64, c_vp, c_vp]),
     "tr_tvm_dual": (c_int, [c_int] * 5 + [c_vp, c_vp, c_f32, c_f32, c_int, c_vp]),

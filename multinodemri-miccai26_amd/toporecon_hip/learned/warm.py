@@ -10,6 +10,17 @@ and the warm start is u0 = c v.  PDHG then runs from u0 to the same
 tolerance as the plain solve, so the network only changes where the
 iterations start, never what they converge to; with no weights, or a shape
 the network was not built for, the caller runs plain PDHG.
+
+``cg_refine`` optionally follows with a few conjugate-gradient steps on the
+data normal equations A^H W A u = A^H W y from u0 (float64 step sizes and
+residual sums on the GPU): CG removes the well-sampled part of the remaining
+error much faster than PDHG's fixed steps do.  ``warm_start`` chains the two.
+
+DEFAULT_CG_STEPS = 3 was chosen on held-out problems of the training size
+(64x96x96, 16 coils, 4 echoes, 6 bins, seeds 9000-9002): it is the largest
+step count whose final objective stayed within +-0.1 % of the plain solve on
+every seed (3 steps: 1.9-2.3x faster to tol 1e-3, +3e-4 to +5e-4; 5 steps:
+4-5x faster, but up to 0.15 % *below* the plain objective).
 """
 
 from __future__ import annotations
@@ -19,6 +30,8 @@ import numpy as np
 from .. import _lib
 
 FIT_BLOCKS = 512
+CG_BLOCKS = 1024
+DEFAULT_CG_STEPS = 3
 
 
 def dc_adjoint(solver, out=None):
@@ -98,3 +111,66 @@ def cnn_warm_start(solver, net, chunk=6):
     _lib.call("tr_cnn_unpack", x.data_ptr(), out.data_ptr(), scale2.data_ptr(), V, B * E, v.data_ptr(),
               net.stream)
     return v.reshape(B, E, *shape)
+
+
+def _host_sum(partial):
+    return float(partial.cpu().numpy().sum())  # block sums added in block order on the host
+
+
+def _normal(solver, x, out, qtmp):
+    """out = A^H W A x over all bins."""
+    s = solver.stream
+    for b in range(solver.B):
+        _lib.call("tr_data_simulate", solver.plan.handle, b, x[b].data_ptr(), solver.maps.data_ptr(), solver.E,
+                  solver.C, qtmp[b].data_ptr(), s)
+        _lib.call("tr_data_adjoint", solver.plan.handle, b, qtmp[b].data_ptr(), solver.dcf[b].data_ptr(),
+                  solver.maps.data_ptr(), solver.E, solver.C, out[b].data_ptr(), 0, s)
+
+
+def cg_refine(solver, x0, steps: int):
+    """`steps` conjugate-gradient iterations on A^H W A u = A^H W y from x0 (returns a new array)."""
+    import torch
+    if steps <= 0:
+        return x0
+    s = solver.stream
+    n = x0.numel()
+    dev = x0.device
+    x = torch.empty_like(x0)
+    _lib.call("tr_cscale", x0.data_ptr(), n, 1.0, x.data_ptr(), s)
+    r = dc_adjoint(solver)
+    qtmp = [torch.empty_like(solver.y[b]) for b in range(solver.B)]
+    ap = torch.empty_like(x)
+    p = torch.empty_like(x)
+    partial = torch.empty(CG_BLOCKS, dtype=torch.float64, device=dev)
+    _normal(solver, x, ap, qtmp)
+    # r = A^H W y - A^H W A x (p is scratch for the x-side update here)
+    _lib.call("tr_cg_update", p.data_ptr(), r.data_ptr(), ap.data_ptr(), ap.data_ptr(), n, 1.0,
+              partial.data_ptr(), CG_BLOCKS, s)
+    rr = _host_sum(partial)
+    _lib.call("tr_cscale", r.data_ptr(), n, 1.0, p.data_ptr(), s)
+    for _ in range(steps):
+        if rr <= 0.0:
+            break
+        _normal(solver, p, ap, qtmp)
+        _lib.call("tr_cdot_re", p.data_ptr(), ap.data_ptr(), n, partial.data_ptr(), CG_BLOCKS, s)
+        pap = _host_sum(partial)
+        if not pap > 0.0:
+            break
+        alpha = rr / pap
+        _lib.call("tr_cg_update", x.data_ptr(), r.data_ptr(), p.data_ptr(), ap.data_ptr(), n, alpha,
+                  partial.data_ptr(), CG_BLOCKS, s)
+        rr_new = _host_sum(partial)
+        _lib.call("tr_cg_direction", p.data_ptr(), r.data_ptr(), n, rr_new / rr, s)
+        rr = rr_new
+    del qtmp, ap, p, r
+    return x
+
+
+def warm_start(solver, net=None, cg_steps: int = DEFAULT_CG_STEPS, chunk=6):
+    """Warm start u0: the network's estimate (zeros without a network), then cg_steps CG steps."""
+    import torch
+    if net is not None:
+        u0 = cnn_warm_start(solver, net, chunk=chunk)
+    else:
+        u0 = torch.zeros(solver.vol_dims(), dtype=torch.complex64, device=solver.device)
+    return cg_refine(solver, u0, cg_steps)

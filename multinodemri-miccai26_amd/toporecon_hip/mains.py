@@ -60,6 +60,8 @@ def build_parser(algorithm: str) -> argparse.ArgumentParser:
         # not a reference flag: start PDHG from the trained CNN's estimate (same tolerance)
         p.add_argument("--warm-start", choices=("none", "learned"), default="none")
         p.add_argument("--warm-start-weights", type=str, default=None)
+        p.add_argument("--warm-start-cg-steps", type=int, default=None,
+                       help="conjugate-gradient data-consistency steps after the CNN (default 3)")
     p.add_argument("--prepared-dir", type=str, default=None)
     p.add_argument("--output-dir", type=str, default=".")
     p.add_argument("--verbose", action="store_true")
@@ -137,6 +139,15 @@ def load_conditioned(args, algorithm):
                 fov=fov, prepared=prepared)
 
 
+def _cg_steps(args):
+    if args.warm_start_cg_steps is not None:
+        if args.warm_start_cg_steps < 0:
+            raise ValueError("--warm-start-cg-steps must be >= 0")
+        return args.warm_start_cg_steps
+    from .learned.warm import DEFAULT_CG_STEPS
+    return DEFAULT_CG_STEPS
+
+
 def _manifest(args, algorithm, E_total, prepared):
     params = {"readout_fraction": args.frac, "num_bins": args.num_bins, "num_echoes": E_total,
               "max_iter": args.max_iter, "tol": args.tol, "fov_scale_zyx": list(args.fov_scale),
@@ -147,7 +158,7 @@ def _manifest(args, algorithm, E_total, prepared):
     elif algorithm == "tvme":
         params.update(lambda_motion=args.lamda_m, lambda_echo=args.lamda_e, acceleration=args.acc)
         if getattr(args, "warm_start", "none") != "none":  # absent in the reference's manifest
-            params.update(warm_start=args.warm_start)
+            params.update(warm_start=args.warm_start, warm_start_cg_steps=_cg_steps(args))
         density = "maximum"
     else:
         params.update(lambda_motion=args.lambda1, lambda_echo_wavelet=args.lambda2,
@@ -200,12 +211,13 @@ def run_main(algorithm: str, argv=None) -> int:
                                 lambda2=args.lambda2, lambda3=args.lambda3, E_total=data["E_total"], e0=0,
                                 e1=data["E_total"], **common)
     if getattr(args, "warm_start", "none") == "learned":
-        from .learned import WarmStartNet, cnn_warm_start, default_weights_path
+        from .learned import WarmStartNet, default_weights_path, warm_start
         path = args.warm_start_weights or default_weights_path()
         if not os.path.exists(path):
             raise FileNotFoundError(f"--warm-start learned: no weights at {path}")
         net = WarmStartNet.load(path, args.device)
-        img = rec.run(init=cnn_warm_start(rec, net), dual_init="residual").cpu().numpy()
+        u0 = warm_start(rec, net, cg_steps=_cg_steps(args))
+        img = rec.run(init=u0, dual_init="residual").cpu().numpy()
     else:
         img = rec.run().cpu().numpy()
     logging.info("Reconstruction done in %.2f seconds (%d iterations).", time.perf_counter() - t0,
