@@ -92,6 +92,10 @@ int tr_plan_release_workspace(tr_plan* plan);
  * calls for bins in different slots may run concurrently on different streams (each slot holds
  * its own oversampled grids). */
 #define TR_PARAM_WORKSPACE_SLOTS 7
+/* TR_PARAM_LINES_XCD_LINES (0..64, default 4): the y and z FFT passes keep runs of this many
+ * whole lines (all channel chunks of each) on one XCD, so the chunks of a grid point's channel
+ * record are fetched and written close together in time (0 = hardware round-robin). */
+#define TR_PARAM_LINES_XCD_LINES 8
 int tr_plan_set_param(tr_plan* plan, int param, int64_t value);
 int tr_plan_get_param(const tr_plan* plan, int param, int64_t* value);
 

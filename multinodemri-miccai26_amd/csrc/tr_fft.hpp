@@ -30,14 +30,14 @@ struct FftAxis {
 __host__ __device__ constexpr int lds_ld(int kc) { return kc >= 4 ? kc + 1 : kc; }
 
 // ---- small DFT kernels (forward, w = exp(-2 pi i / R)) --------------------
-__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }  // -i * a
+__host__ __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }  // -i * a
 
 template <int R>
 struct Dft;
 
 template <>
 struct Dft<2> {
-  __device__ static __forceinline__ void run(float2 (&v)[2], const float2*, int) {
+  __host__ __device__ static __forceinline__ void run(float2 (&v)[2], const float2*, int) {
     const float2 a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -46,7 +46,7 @@ struct Dft<2> {
 
 template <>
 struct Dft<3> {
-  __device__ static __forceinline__ void run(float2 (&v)[3], const float2*, int) {
+  __host__ __device__ static __forceinline__ void run(float2 (&v)[3], const float2*, int) {
     const float s = 0.86602540378443864676f;  // sin(2 pi / 3)
     const float2 t1 = cadd(v[1], v[2]);
     const float2 d = csub(v[1], v[2]);
@@ -59,7 +59,7 @@ struct Dft<3> {
 
 template <>
 struct Dft<4> {
-  __device__ static __forceinline__ void run(float2 (&v)[4], const float2*, int) {
+  __host__ __device__ static __forceinline__ void run(float2 (&v)[4], const float2*, int) {
     const float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
     const float2 c = cadd(v[1], v[3]), d = mul_mi(csub(v[1], v[3]));
     v[0] = cadd(a, c);
@@ -71,7 +71,7 @@ struct Dft<4> {
 
 template <>
 struct Dft<5> {
-  __device__ static __forceinline__ void run(float2 (&v)[5], const float2*, int) {
+  __host__ __device__ static __forceinline__ void run(float2 (&v)[5], const float2*, int) {
     const float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
     const float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
     const float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
@@ -93,7 +93,7 @@ struct Dft<5> {
 
 template <>
 struct Dft<8> {
-  __device__ static __forceinline__ void run(float2 (&v)[8], const float2* tw, int L) {
+  __host__ __device__ static __forceinline__ void run(float2 (&v)[8], const float2* tw, int L) {
     const float h = 0.70710678118654752440f;
     float2 e[4] = {v[0], v[2], v[4], v[6]};
     float2 o[4] = {v[1], v[3], v[5], v[7]};
@@ -114,7 +114,7 @@ struct Dft<8> {
 // Generic odd prime radix via the line's twiddle table: w_R^m = tw[m L / R].
 template <int R>
 struct Dft {
-  __device__ static __forceinline__ void run(float2 (&v)[R], const float2* tw, int L) {
+  __host__ __device__ static __forceinline__ void run(float2 (&v)[R], const float2* tw, int L) {
     const int step = L / R;
     float2 out[R];
 #pragma unroll

@@ -111,6 +111,7 @@ struct tr_plan {
   // one XCD by the gather kernels (0 = hardware round-robin order)
   int interp_xcd_chunk = 64;
   int spread_xcd_chunk = 0;
+  int lines_xcd_lines = 4;  // y / z passes: whole lines per XCD run (0 = round-robin)
   int interp_kgroups = 0;  // 1: one 64-channel group per launch row (group-major order)
   // k-space support of all bins' taps: per (y, x) column of the oversampled
   // grid (natural order) the window (start, len) of z positions any tap
@@ -151,6 +152,7 @@ struct FwdXArgs {
 
 struct LinesArgs {
   FftAxis ax;
+  unsigned xcd_chunk;  // workgroups per XCD run (whole lines of channel chunks), 0 = round-robin
   const float2* tw;
   float2* data;
   int K;

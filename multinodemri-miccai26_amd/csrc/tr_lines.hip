@@ -1,5 +1,6 @@
 // FFT passes along y and z of the channel-innermost grid.
 
+#include "tr_fft2.hpp"
 #include "tr_internal.hpp"
 
 namespace tr {
@@ -8,7 +9,8 @@ namespace tr {
 // Generic pass along y or z: load a (possibly windowed) line of KC channels,
 // FFT (or inverse), store a (possibly windowed) set of positions.
 // Block = (channel chunk, line) with the chunk fastest, so concurrent blocks
-// cover contiguous channel runs of the same positions.
+// cover contiguous channel runs of the same positions; with xcd_chunk, runs of
+// whole lines stay on one XCD.
 // ---------------------------------------------------------------------------
 template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
@@ -20,8 +22,9 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
   const int tid = threadIdx.x;
   const int ch = tid % KC, w = tid / KC;
   const int nchunk = (a.K + KC - 1) / KC;
-  const int line = blockIdx.x / nchunk;
-  const int chunk = blockIdx.x - line * nchunk;
+  const unsigned bid = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
+  const int line = bid / nchunk;
+  const int chunk = bid - line * nchunk;
   const int o = line / a.n_inner;
   const int in = line - o * a.n_inner;
   // support pruning acts on the k-space side: the stores of a forward pass,
@@ -32,7 +35,7 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
     if (lw.y == 0) {  // empty line: nothing live to load; its outputs are zero
       if (a.zero_empty) {
         const int nchunk0 = (a.K + KC - 1) / KC;
-        const int k00 = (blockIdx.x % nchunk0) * KC;
+        const int k00 = (bid % nchunk0) * KC;
         if (k00 + ch < a.K) {
           const int Ls = LC > 0 ? LC : a.ax.L;
           const int ob0 = a.outer_win_n > 0 ? pmod(o - a.outer_win_n / 2, a.outer_win_g) : o;
@@ -52,6 +55,47 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
   const bool live = k0 + ch < a.K;
   float2* base = a.data + ob * a.outer_stride + in * a.inner_stride + k0 + ch;
   for (int i = tid; i < L; i += kBlock) tw[i] = a.tw[i];
+  if constexpr (two_level_ok<KC, LC>()) {
+    // two-level register FFT: positions p = w + N2 r in, p = w + N1 k2 out
+    constexpr int N1 = split_n1(LC), N2 = LC / N1;
+    const int lh = a.load_n / 2, sh = a.store_n / 2;
+    const bool wl = a.load_n < LC, ws = a.store_n < LC;
+    // Masked loads (padding and pruned positions are never fetched), all issued before any is
+    // used so that they are in flight together.
+    float2 v[N1];
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int p = w + N2 * r;
+      // loaded positions are (s - n//2) mod L for s < load_n; the rest are zero padding
+      bool ok = w < N2 && live && (!wl || pmod(p + lh, LC) < a.load_n);
+      if (a.inverse) ok = ok && live_pos(p);
+      v[r] = make_float2(0.f, 0.f);
+      if (ok) v[r] = base[p * a.pos_stride];
+    }
+#pragma unroll
+    for (int r = 0; r < N1; ++r) keep_loaded(v[r]);
+    if (a.inverse) {
+#pragma unroll
+      for (int r = 0; r < N1; ++r) v[r].y = -v[r].y;
+    }
+    __syncthreads();  // twiddle table
+    if (w < N2) two_level_pass1<LC>(v, buf, T::ld, ch, w, tw);
+    __syncthreads();
+    if (w < N1 && live) {
+      float2 z[N2];
+      two_level_pass2<LC>(z, buf, T::ld, ch, w);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int p = w + N1 * k2;
+        if ((!ws || pmod(p + sh, LC) < a.store_n) && (a.inverse || live_pos(p))) {
+          float2 x = z[k2];
+          if (a.inverse) x.y = -x.y;
+          base[p * a.pos_stride] = x;
+        }
+      }
+    }
+    return;
+  }
   const bool win_load = a.load_n < L;
   float2 v[T::PER];
 #pragma unroll

@@ -43,6 +43,7 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
   const int nchunk = (K + kc - 1) / kc;
   {  // y pass over the nz occupied z-planes, windowed load along y
     LinesArgs a;
+    a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[1];
     a.tw = p->tw[1];
     a.data = p->work;
@@ -67,6 +68,7 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
   }
   {  // z pass over every (y, x) line, windowed load along z
     LinesArgs a;
+    a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[0];
     a.tw = p->tw[0];
     a.data = p->work;
@@ -104,6 +106,7 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
   const int nchunk = (K + kc - 1) / kc;
   {
     LinesArgs a;
+    a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[0];
     a.tw = p->tw[0];
     a.data = p->work;
@@ -129,6 +132,7 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
   }
   {
     LinesArgs a;
+    a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[1];
     a.tw = p->tw[1];
     a.data = p->work;

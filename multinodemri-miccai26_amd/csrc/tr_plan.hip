@@ -622,6 +622,10 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
       if (value < 1 || value > kMaxSlots) TR_FAIL(kInvalidArgument, "workspace slots outside [1, 8]");
       p->nslots = (int)value;
       return kOk;
+    case TR_PARAM_LINES_XCD_LINES:
+      if (value < 0 || value > 64) TR_FAIL(kInvalidArgument, "lines per XCD run outside [0, 64]");
+      p->lines_xcd_lines = (int)value;
+      return kOk;
     case TR_PARAM_SUPPORT_POINTS:
     case TR_PARAM_LIVE_COLUMNS:
       TR_FAIL(kInvalidArgument, "read-only plan parameter " + std::to_string(param));
@@ -637,6 +641,7 @@ int tr_plan_get_param(const tr_plan* p, int param, int64_t* value) {
     case TR_PARAM_PRUNE: *value = p->prune; return kOk;
     case TR_PARAM_INTERP_KGROUPS: *value = p->interp_kgroups; return kOk;
     case TR_PARAM_WORKSPACE_SLOTS: *value = p->nslots; return kOk;
+    case TR_PARAM_LINES_XCD_LINES: *value = p->lines_xcd_lines; return kOk;
     case TR_PARAM_SUPPORT_POINTS: *value = p->support_points; return kOk;
     case TR_PARAM_LIVE_COLUMNS: *value = p->live_cols; return kOk;
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));

@@ -1,6 +1,7 @@
 // Apodise / pad / FFT along x (forward) and inverse FFT / crop / coil sum
 // along x (adjoint), one image row per workgroup.
 
+#include "tr_fft2.hpp"
 #include "tr_internal.hpp"
 
 namespace tr {
@@ -31,6 +32,64 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
   for (int i = tid; i < L; i += kBlock) tw[i] = a.tw[i];
   const int64_t rowvox = ((int64_t)z * ny + y) * nx;
   const float azy = a.apod_z[z] * a.apod_y[y] * a.scale;
+  if constexpr (two_level_ok<KC, LC>()) {
+    // two-level register FFT: worker w < N2 holds padded positions p = w + N2 r, whose image
+    // column is x = (p + nx//2) mod gx (zero padding where x >= nx); the next echo's row is
+    // loaded while the current one is transformed
+    constexpr int N1 = split_n1(LC), N2 = LC / N1;
+    // Every load below reads a clamped, in-bounds address and the padding is applied with a
+    // select afterwards, so no load sits in a per-lane branch and all of them are in flight at once.
+    const int cc = min(c, a.C - 1);
+    float2 sm[N1], u[N1];
+    float ap[N1];
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int xc = min(pmod(w + N2 * r + nx / 2, gx), nx - 1);
+      ap[r] = a.apod_x[xc];
+      u[r] = a.img[rowvox + xc];
+    }
+    if (a.maps) {
+#pragma unroll
+      for (int r = 0; r < N1; ++r) sm[r] = a.maps[(int64_t)cc * N + rowvox + min(pmod(w + N2 * r + nx / 2, gx), nx - 1)];
+    } else {
+#pragma unroll
+      for (int r = 0; r < N1; ++r) sm[r] = make_float2(1.f, 0.f);
+    }
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      keep_loaded(sm[r]);
+      keep_loaded(ap[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+      const int x = pmod(w + N2 * r + nx / 2, gx);
+      sm[r] = (w < N2 && x < nx && live) ? cscale(sm[r], azy * ap[r]) : make_float2(0.f, 0.f);
+    }
+    const int zp = pmod(z - nz / 2, gz), yp = pmod(y - ny / 2, gy);
+    const int E = a.K / a.C;
+    for (int e = 0; e < E; ++e) {
+      float2 v[N1];
+      // sm is zero at padding positions, so the product is the zero-padded, coil-weighted row
+#pragma unroll
+      for (int r = 0; r < N1; ++r) v[r] = cmul(u[r], sm[r]);
+      if (e + 1 < E) {
+#pragma unroll
+        for (int r = 0; r < N1; ++r)
+          u[r] = a.img[(int64_t)(e + 1) * N + rowvox + min(pmod(w + N2 * r + nx / 2, gx), nx - 1)];
+      }
+      __syncthreads();  // twiddles / the previous echo's pass-2 reads
+      if (w < N2) two_level_pass1<LC>(v, buf, T::ld, ch, w, tw);
+      __syncthreads();
+      if (w < N1 && live) {
+        float2 zz[N2];
+        two_level_pass2<LC>(zz, buf, T::ld, ch, w);
+        float2* dst = a.grid + (((int64_t)zp * gy + yp) * gx) * a.K + (int64_t)e * a.C;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) dst[(int64_t)(w + N1 * k2) * a.K + c] = zz[k2];
+      }
+    }
+    return;
+  }
   // map row of this thread's coil, pre-multiplied by the apodisation
   float2 sm[T::PER];
 #pragma unroll
@@ -99,6 +158,76 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
   const int zp = pmod(z - nz / 2, gz), yp = pmod(y - ny / 2, gy);
   const float2* src = a.grid + (((int64_t)zp * gy + yp) * gx) * a.K;
   const int64_t rowvox = ((int64_t)z * ny + y) * nx;
+  if constexpr (two_level_ok<KC, LC>()) {
+    // two-level register FFT; pass 2 multiplies its outputs by conj(map) and writes them to
+    // LDS by image column, where thread x adds the chunk's coils in increasing order (the same
+    // products in the same order as the stage-by-stage path)
+    constexpr int N1 = split_n1(LC), N2 = LC / N1;
+    float2 acc[kMaxE];
+#pragma unroll
+    for (int e = 0; e < kMaxE; ++e) acc[e] = make_float2(0.f, 0.f);
+    for (int c0 = 0; c0 < a.C; c0 += KC) {
+      const int cn = min(KC, a.C - c0);
+      // clamped, unconditional loads (see the forward pass); results are selected afterwards
+      const int chc = c0 + min(ch, cn - 1);
+      float2 mp[N2];
+      if (a.maps) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+          mp[k2] = a.maps[(int64_t)chc * N + rowvox + min(pmod(w + N1 * k2 + nx / 2, gx), nx - 1)];
+      } else {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) mp[k2] = make_float2(1.f, 0.f);
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) keep_loaded(mp[k2]);
+#pragma unroll
+      for (int e = 0; e < kMaxE; ++e) {
+        if (e < E) {
+          float2 v[N1];
+          const bool lv = w < N2 && ch < cn;
+#pragma unroll
+          for (int r = 0; r < N1; ++r) v[r] = src[(int64_t)min(w + N2 * r, LC - 1) * a.K + (int64_t)e * a.C + chc];
+#pragma unroll
+          for (int r = 0; r < N1; ++r) keep_loaded(v[r]);
+#pragma unroll
+          for (int r = 0; r < N1; ++r) v[r] = lv ? cconj(v[r]) : make_float2(0.f, 0.f);
+          __syncthreads();  // twiddles / the previous accumulation's reads
+          if (w < N2) two_level_pass1<LC>(v, buf, T::ld, ch, w, tw);
+          __syncthreads();
+          float2 zz[N2];
+          if (w < N1) two_level_pass2<LC>(zz, buf, T::ld, ch, w);
+          __syncthreads();  // every tile read before the products overwrite it
+          if (w < N1 && ch < cn) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+              const int x = pmod(w + N1 * k2 + nx / 2, gx);
+              if (x < nx) {
+                const float2 r = cconj(zz[k2]);
+                buf[x * T::ld + ch] = a.maps ? cmulc(mp[k2], r) : r;
+              }
+            }
+          }
+          __syncthreads();
+          if (tid < nx)
+            for (int cc = 0; cc < cn; ++cc) acc[e] = cadd(acc[e], buf[tid * T::ld + cc]);
+        }
+      }
+    }
+    if (tid < nx) {
+      const int x = tid;
+      const float sc = (a.apod_z[z] * a.apod_y[y] * a.apod_x[x]) * a.scale;
+#pragma unroll
+      for (int e = 0; e < kMaxE; ++e) {
+        if (e < E) {
+          const float2 r = cscale(acc[e], sc);
+          float2* o = a.out + (int64_t)e * N + rowvox + x;
+          *o = a.accumulate ? cadd(*o, r) : r;
+        }
+      }
+    }
+    return;
+  }
   // Host guarantees E <= kMaxE.
   float2 acc[kMaxE][kSlots];
 #pragma unroll

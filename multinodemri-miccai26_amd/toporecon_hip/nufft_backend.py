@@ -120,7 +120,7 @@ class NufftPlan:
         _lib.call("tr_plan_release_workspace", self.handle)
 
     PARAMS = {"interp_xcd_chunk": 1, "spread_xcd_chunk": 2, "prune": 3, "support_points": 4,
-              "live_columns": 5, "interp_kgroups": 6, "workspace_slots": 7}
+              "live_columns": 5, "interp_kgroups": 6, "workspace_slots": 7, "lines_xcd_lines": 8}
 
     def set_param(self, name: str, value: int):
         """Set a launch parameter (block order only; results are unchanged)."""

@@ -91,6 +91,29 @@ def test_batched_coils_echoes(torch_mod, E, C):
     assert _rel(adj, ref) < TOL
 
 
+@pytest.mark.parametrize("shape", [(8, 16, 192), (8, 192, 8), (80, 8, 8), (32, 48, 64), (16, 96, 48)])
+def test_two_level_fft_lengths(torch_mod, shape):
+    """16-coil / 32-channel batches on grids whose lengths (240, 100, 80, 60, 40, 120) take the
+    two-level register FFT in the x, y and z passes, forward and density-weighted adjoint."""
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    print("two-level", shape, flush=True)
+    E, C, M = 2, 16, 3000
+    rng = np.random.default_rng(sum(shape))
+    coords = [_coords(rng, shape, M)]
+    be = HipNufftBackend(shape, coords, 0)
+    img = _cplx(rng, (E,) + shape)
+    maps = _cplx(rng, (C,) + shape)
+    out = be.forward_batch(img, maps, 0).cpu().numpy()
+    ref = on.nufft(img[:, None].astype(np.complex128) * maps[None], coords[0])
+    assert _rel(out, ref) < TOL, _rel(out, ref)
+    k = _cplx(rng, (E, C, M))
+    dcf = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    adj = be.adjoint_batch(k, maps, 0, dcf=dcf).cpu().numpy()
+    x = on.nufft_adjoint(k.astype(np.complex128) * dcf, coords[0], shape)
+    ref = np.sum(np.conj(maps)[None] * x, axis=1)
+    assert _rel(adj, ref) < TOL, _rel(adj, ref)
+
+
 def test_adjointness_and_determinism(torch_mod):
     from toporecon_hip.nufft_backend import HipNufftBackend
     shape = (12, 16, 14)

@@ -90,11 +90,13 @@ def test_launch_params_and_autotune_keep_results_bitwise(torch_mod):
     acq, ksp, prob = make_problem(seed=11)
     base = _hip_solver("tvme", acq, ksp, 3, 0.0, True)
     ref = base.run().cpu().numpy()
-    for ci, cs, kg in [(0, 0, 0), (1, 1, 1), (16, 64, 0), (1024, 4096, 1)]:
+    for ci, cs, kg, xl in [(0, 0, 0, 0), (1, 1, 1, 1), (16, 64, 0, 4), (1024, 4096, 1, 64)]:
         base.plan.set_param("interp_xcd_chunk", ci)
         base.plan.set_param("spread_xcd_chunk", cs)
         base.plan.set_param("interp_kgroups", kg)
+        base.plan.set_param("lines_xcd_lines", xl)
         assert base.plan.get_param("interp_xcd_chunk") == ci
+        assert base.plan.get_param("lines_xcd_lines") == xl
         out = base.run().cpu().numpy()
         assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (ci, cs)
     for n in (3, 2, 1):  # bins on several streams, one grid workspace per stream
@@ -107,6 +109,8 @@ def test_launch_params_and_autotune_keep_results_bitwise(torch_mod):
     assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
     with pytest.raises(Exception):
         base.plan.set_param("interp_xcd_chunk", 5000)
+    with pytest.raises(Exception):
+        base.plan.set_param("lines_xcd_lines", 65)
 
 
 @pytest.mark.gpu
