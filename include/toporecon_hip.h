@@ -66,8 +66,9 @@ int tr_graph_destroy(tr_graph* graph);
 int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
                    const int64_t* samples_per_bin, void* stream);
 int tr_plan_destroy(tr_plan* plan);
-/* info[0..2] image dims, [3..5] grid dims, [6] bins, [7] M_b, [8] nnz_b, [9] channels of the
- * workspace last used. */
+/* info (12 values): [0..2] image dims, [3..5] grid dims, [6] bins, [7] M_b, [8] nnz_b, [9]
+ * channels of the workspace last used, [10] distinct grid points under bin b's taps, [11] grid
+ * points of the plan's pruned support (all bins). */
 int tr_plan_info(const tr_plan* plan, int bin, int64_t* info);
 int tr_plan_release_workspace(tr_plan* plan);
 

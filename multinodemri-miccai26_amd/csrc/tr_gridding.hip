@@ -333,13 +333,74 @@ __global__ void __launch_bounds__(256) k_spread_combine(SpreadArgs a) {
     a.grid[(((int64_t)z * gy + y) * gx + x) * K + k] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// Distinct grid points read by a bin's interpolation (its taps' union), for
+// the algorithmic byte count of the gather kernels: mark every tap point with
+// a byte store (all stores write 1, so no ordering is needed), then count.
+// ---------------------------------------------------------------------------
+__global__ void k_mark_taps(const TapRec* tap, int64_t M, GridGeom gg, unsigned char* mark) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int gz = gg.g[0], gy = gg.g[1], gx = gg.g[2];
+  const TapRec r = tap[m];
+  const int nx = (r.n >> 8) & 15, ny = (r.n >> 4) & 15;
+  const int nz = (nx == 0 || ny == 0) ? 0 : (r.n & 15);
+  for (int dz = 0; dz < nz; ++dz) {
+    const int pz = wrap2(r.p0[0] + dz, gz);
+    for (int dy = 0; dy < ny; ++dy) {
+      const int py = wrap2(r.p0[1] + dy, gy);
+      for (int dx = 0; dx < nx; ++dx) mark[((int64_t)pz * gy + py) * gx + wrap2(r.p0[2] + dx, gx)] = 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_count_marks(const unsigned char* mark, int64_t n, int64_t* partial) {
+  int64_t c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += mark[i];
+  __shared__ int64_t sh[256];
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+int count_touched(const tr_plan* p, BinData& b, hipStream_t s) {
+  b.touched = 0;
+  if (b.M == 0) return kOk;
+  constexpr int kParts = 256;
+  unsigned char* mark = nullptr;
+  int64_t* part = nullptr;
+  TR_HIP(hipMalloc(&mark, (size_t)p->G));
+  if (hipMalloc(&part, sizeof(int64_t) * kParts) != hipSuccess) {
+    (void)hipFree(mark);
+    TR_FAIL(kOutOfMemory, "touched-point counters");
+  }
+  int64_t host[kParts];
+  hipError_t e = hipMemsetAsync(mark, 0, (size_t)p->G, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mark_taps, dim3(grid1d(b.M, 256)), dim3(256), 0, s, b.tap, b.M, geom(p), mark);
+    hipLaunchKernelGGL(k_count_marks, dim3(kParts), dim3(256), 0, s, mark, p->G, part);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(host, part, sizeof(host), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(mark);
+  (void)hipFree(part);
+  TR_HIP(e);
+  for (int i = 0; i < kParts; ++i) b.touched += host[i];
+  return kOk;
+}
+
 int launch_interp(const InterpArgs& a, hipStream_t s) {
   const int K = a.K;
-  const double G = (double)a.gg.g[0] * a.gg.g[1] * a.gg.g[2];
   const double per_sample = a.mode == kInterpDual ? 24.0 * K
                             : a.mode == kInterpResid ? 16.0 * K
                             : (a.mode == kInterpPlain ? 8.0 * K + 4.0 : 8.0 * K);
-  ProfScope ps(kKInterp, 8.0 * K * G + a.M * (12.0 + per_sample), s);
+  ProfScope ps(kKInterp, 8.0 * K * (double)a.touched + a.M * (12.0 + per_sample), s);
   const int kci = K >= 64 ? 64 : pow2ceil(K);
   const int nci = (K + kci - 1) / kci;
   const int spw = 64 / kci;

@@ -69,6 +69,7 @@ struct BinData {
   int* seg_j0 = nullptr;         // first pair of each segment
   int* heavy_row = nullptr;      // heavy cluster ids
   int* heavy_seg_ptr = nullptr;  // nheavy + 1
+  int64_t touched = 0;           // distinct grid points under the bin's taps
 };
 
 }  // namespace tr
@@ -194,6 +195,7 @@ enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2, kInte
 
 struct InterpArgs {
   GridGeom gg;
+  int64_t touched;  // distinct grid points read (algorithmic byte count only)
   int xcd_chunk;
   int kgroups;
   const TapRec* tap;
@@ -304,6 +306,7 @@ int ensure_work(tr_plan* p, int64_t K);
 int build_support(tr_plan* p, hipStream_t s);
 void use_slot(tr_plan* p, int slot);
 int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s);
+int count_touched(const tr_plan* p, BinData& b, hipStream_t s);
 void free_plan(tr_plan* p);
 int check_plan_bin(const tr_plan* p, int bin);
 

@@ -199,6 +199,7 @@ int tr_nufft_forward(tr_plan* p, int bin, const tr_c64* img, const tr_c64* maps,
   a.xcd_chunk = p->interp_xcd_chunk;
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
+  a.touched = b.touched;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -265,6 +266,7 @@ int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* m
   a.xcd_chunk = p->interp_xcd_chunk;
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
+  a.touched = b.touched;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -289,6 +291,7 @@ int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   a.xcd_chunk = p->interp_xcd_chunk;
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
+  a.touched = b.touched;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -310,6 +313,7 @@ int tr_data_residual(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   a.xcd_chunk = p->interp_xcd_chunk;
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
+  a.touched = b.touched;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;

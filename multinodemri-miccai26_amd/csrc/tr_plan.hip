@@ -544,7 +544,8 @@ int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float
       free_plan(p);
       TR_FAIL(kInvalidArgument, "negative sample count");
     }
-    const int r = build_bin(p, p->bins[b], coords[b], samples_per_bin[b], s);
+    int r = build_bin(p, p->bins[b], coords[b], samples_per_bin[b], s);
+    if (!r) r = count_touched(p, p->bins[b], s);
     if (r) {
       std::string msg = last_error();
       free_plan(p);
@@ -584,6 +585,8 @@ int tr_plan_info(const tr_plan* p, int bin, int64_t* info) {
     info[7] = info[8] = -1;
   }
   info[9] = p->work_k;
+  info[10] = (bin >= 0 && bin < (int)p->bins.size()) ? p->bins[bin].touched : -1;
+  info[11] = p->support_points;
   return kOk;
 }
 

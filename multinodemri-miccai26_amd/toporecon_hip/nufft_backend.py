@@ -106,7 +106,7 @@ class NufftPlan:
         self.grid_shape = tuple(int(v) for v in info[3:6])
 
     def info(self, bin_index: int):
-        out = (ctypes.c_int64 * 10)()
+        out = (ctypes.c_int64 * 12)()
         _lib.call("tr_plan_info", self.handle, int(bin_index), out)
         return list(out)
 
@@ -115,6 +115,10 @@ class NufftPlan:
 
     def nnz(self, bin_index: int) -> int:
         return int(self.info(bin_index)[8])
+
+    def touched_points(self, bin_index: int) -> int:
+        """Distinct grid points under the taps of bin ``bin_index`` (what its interpolation reads)."""
+        return int(self.info(bin_index)[10])
 
     def release_workspace(self):
         _lib.call("tr_plan_release_workspace", self.handle)

@@ -114,6 +114,23 @@ def test_two_level_fft_lengths(torch_mod, shape):
     assert _rel(adj, ref) < TOL, _rel(adj, ref)
 
 
+def test_touched_points_match_oracle_taps(torch_mod):
+    """The plan's per-bin count of distinct grid points under the taps (the interpolation's
+    algorithmic grid bytes) equals the support of the oracle's gridding of all-ones samples
+    (every Kaiser-Bessel tap weight, including |x| = 1, is positive)."""
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    shape = (12, 16, 14)
+    rng = np.random.default_rng(17)
+    coords = [_coords(rng, shape, 3000), _coords(rng, shape, 40)]
+    be = HipNufftBackend(shape, coords, 0)
+    gshape = on.oversamp_shape(shape)
+    beta32 = float(np.float32(on.kb_beta()))
+    for b, c in enumerate(coords):
+        g = on.gridding(np.ones(len(c), np.complex128), on.scale_coord(c, shape), gshape, beta32)
+        assert be.plan.touched_points(b) == int(np.count_nonzero(g)), b
+        assert be.plan.touched_points(b) <= be.plan.info(b)[11] <= int(np.prod(gshape))
+
+
 def test_adjointness_and_determinism(torch_mod):
     from toporecon_hip.nufft_backend import HipNufftBackend
     shape = (12, 16, 14)
