@@ -120,6 +120,95 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The same gather with 16-byte loads: lane l of the wave owns the channel pair
+// (kbase + 2l, kbase + 2l + 1) of a group of G = gridDim.y-th of the channels
+// (G even, at most 128), so one load instruction moves up to 1 KB of a grid
+// point's channel record.  One sample per wave (record in scalar registers);
+// channel groups are launch rows (group-major).  Per channel the products are
+// added in the same (dz, dy, dx) order as k_interp, so results are identical.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_interp_v4(InterpArgs a, int G) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
+  const int64_t m = (int64_t)blk * 4 + wave;
+  if (m >= a.M) return;
+  const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
+  const int K = a.K;
+  const int k2 = blockIdx.y * G + 2 * lane;         // first channel of this lane's pair
+  const bool live = 2 * lane < G && k2 < K;
+  const int kk = min(k2, K - 2) >> 1;               // float4 index of the (clamped) pair
+  const TapRec& r = a.tap[m];
+  const int packed = r.n;
+  const int nx = (packed >> 8) & 15, ny = (packed >> 4) & 15;
+  const int nz = (nx == 0 || ny == 0) ? 0 : (packed & 15);
+  const int p0z = r.p0[0], p0y = r.p0[1], p0x = r.p0[2];
+  const int K2 = K >> 1;
+  int px[5];
+#pragma unroll
+  for (int dx = 0; dx < 5; ++dx) px[dx] = wrap2(p0x + dx, gx);
+  const float4* grid4 = reinterpret_cast<const float4*>(a.grid);
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int dz = 0; dz < nz; ++dz) {
+    const int pz = wrap2(p0z + dz, gz);
+    const float wz = r.w[dz];
+    float4 v[4][4];
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+      const float4* row = grid4 + (((int64_t)pz * gy + wrap2(p0y + dy, gy)) * gx) * K2;
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) v[dy][dx] = row[(int64_t)px[dx] * K2 + kk];
+    }
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+      if (dy == 4 && ny < 5) break;
+      const float wzy = wz * r.w[5 + dy];
+      const float4* row = grid4 + (((int64_t)pz * gy + wrap2(p0y + dy, gy)) * gx) * K2;
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) {
+        const float w = wzy * r.w[10 + dx];
+        const float4 u = dy < 4 ? v[dy][dx] : row[(int64_t)px[dx] * K2 + kk];
+        acc.x += w * u.x;
+        acc.y += w * u.y;
+        acc.z += w * u.z;
+        acc.w += w * u.w;
+      }
+      if (nx == 5) {
+        const float w = wzy * r.w[14];
+        const float4 u = row[(int64_t)px[4] * K2 + kk];
+        acc.x += w * u.x;
+        acc.y += w * u.y;
+        acc.z += w * u.z;
+        acc.w += w * u.w;
+      }
+    }
+  }
+  if (!live) return;
+  const float2 a0 = make_float2(acc.x, acc.y), a1 = make_float2(acc.z, acc.w);
+  if (a.mode == kInterpPlain) {
+    const int64_t o = a.perm[m];
+    a.out[(int64_t)k2 * a.M + o] = a0;
+    a.out[(int64_t)(k2 + 1) * a.M + o] = a1;
+  } else {
+    float4* q4 = reinterpret_cast<float4*>(a.q) + m * K2 + (k2 >> 1);
+    if (a.mode == kInterpDual) {
+      const float4 yy = reinterpret_cast<const float4*>(a.y)[m * K2 + (k2 >> 1)];
+      const float4 qo = *q4;
+      const float2 r0 = csub(a0, make_float2(yy.x, yy.y)), r1 = csub(a1, make_float2(yy.z, yy.w));
+      const float t0x = qo.x + a.sigma * r0.x, t0y = qo.y + a.sigma * r0.y;
+      const float t1x = qo.z + a.sigma * r1.x, t1y = qo.w + a.sigma * r1.y;
+      *q4 = make_float4(t0x / a.one_plus_sigma, t0y / a.one_plus_sigma, t1x / a.one_plus_sigma,
+                        t1y / a.one_plus_sigma);
+    } else if (a.mode == kInterpResid) {
+      const float4 yy = reinterpret_cast<const float4*>(a.y)[m * K2 + (k2 >> 1)];
+      *q4 = make_float4(a0.x - yy.x, a0.y - yy.y, a1.x - yy.z, a1.y - yy.w);
+    } else {
+      *q4 = make_float4(a0.x, a0.y, a1.x, a1.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic adjoint spreading over clusters of kClZ x kClY x kClX grid
 // points.  A wave owns one cluster and KCI channels (lanes: KCI channels x
 // 64/KCI point slots) and keeps the cluster's sums in registers while it walks
@@ -406,6 +495,15 @@ int launch_interp(const InterpArgs& a, hipStream_t s) {
   const int spw = 64 / kci;
   const unsigned blocks = grid1d(a.M, 4 * spw);
   if (a.M == 0) return kOk;
+  const bool aligned16 = (((uintptr_t)a.grid | (uintptr_t)a.q | (uintptr_t)a.y) & 15) == 0;
+  if (a.kgroups == 2 && K % 2 == 0 && K >= 96 && aligned16) {  // 16-byte loads, groups of <= 128
+    const int ng = (K + 127) / 128;
+    int G = (K + ng - 1) / ng;
+    G += G & 1;
+    hipLaunchKernelGGL(k_interp_v4, dim3(grid1d(a.M, 4), ng), dim3(256), 0, s, a, G);
+    TR_LAUNCH_CHECK();
+    return kOk;
+  }
   if (a.kgroups && kci == 64) {  // one 64-channel group per grid row, groups outermost
     hipLaunchKernelGGL((k_interp<64, 1>), dim3(blocks, nci), dim3(256), 0, s, a);
     TR_LAUNCH_CHECK();

@@ -618,7 +618,7 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
       p->prune = (int)value;
       return kOk;
     case TR_PARAM_INTERP_KGROUPS:
-      if (value != 0 && value != 1) TR_FAIL(kInvalidArgument, "interp channel groups is 0 or 1");
+      if (value < 0 || value > 2) TR_FAIL(kInvalidArgument, "interp channel groups is 0, 1 or 2");
       p->interp_kgroups = (int)value;
       return kOk;
     case TR_PARAM_WORKSPACE_SLOTS:

@@ -21,7 +21,7 @@ import os
 from . import _lib
 
 INTERP_CANDIDATES = (0, 16, 64, 256, 1024)
-INTERP_KGROUPS = (0, 1)  # all channels per block / one 64-channel group per launch row
+INTERP_KGROUPS = (0, 1, 2)  # all channels per block / 64-channel groups / 16-byte loads, <=128-channel groups
 SPREAD_CANDIDATES = (0, 64)
 
 _cache: dict = {}

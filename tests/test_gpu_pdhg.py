@@ -114,6 +114,22 @@ def test_launch_params_and_autotune_keep_results_bitwise(torch_mod):
 
 
 @pytest.mark.gpu
+def test_interp_channel_group_modes_bitwise(torch_mod):
+    """96 channels (16 coils x 6 echoes): the interpolation's channel-group modes (all channels
+    per block, 64-channel groups, 16-byte loads over 96-channel groups) give the same bits."""
+    acq, ksp, prob = make_problem(shape=(10, 12, 14), coils=16, echoes=6, bins=2, seed=13)
+    s = _hip_solver("tvme", acq, ksp, 3, 0.0, True)
+    outs = []
+    for kg in (0, 1, 2):
+        s.plan.set_param("interp_kgroups", kg)
+        outs.append(s.run().cpu().numpy())
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
+    o_ubar, _, _, _ = _oracle_run(prob, "tvme", 3, True)
+    assert rel(outs[0], o_ubar) < 2e-5
+
+
+@pytest.mark.gpu
 def test_learned_warm_start_gains_over_plain(torch_mod):
     """The committed network, on a held-out problem of the size it was trained at
     (64x96x96, 16 coils, 4 echoes, 6 bins; the training seeds are 1000-1083): the
