@@ -209,6 +209,106 @@ __global__ void __launch_bounds__(256) k_interp_v4(InterpArgs a, int G) {
 }
 
 // ---------------------------------------------------------------------------
+// Grouped gather: one wave per group of up to kTapGroup consecutive sorted
+// samples with identical taps (same first tap and tap counts; dense k-space
+// regions hold many), 16-byte loads as k_interp_v4.  Each tap is loaded once
+// for the group and added into every member's sums with the member's own
+// weights, in the same (dz, dy, dx) order and with the same products as the
+// per-sample kernels, so every sample's result is bit-identical to theirs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void acc4(float4& acc, float w, const float4& u) {
+  acc.x += w * u.x;
+  acc.y += w * u.y;
+  acc.z += w * u.z;
+  acc.w += w * u.w;
+}
+
+__global__ void __launch_bounds__(256) k_interp_grp(InterpArgs a, int G) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
+  const int64_t gi = (int64_t)blk * 4 + wave;
+  if (gi >= a.ngroups) return;
+  const int m0 = a.grp[gi];
+  const int gs = a.grp[gi + 1] - m0;  // 1 .. kTapGroup, wave-uniform
+  const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
+  const int K = a.K;
+  const int k2 = blockIdx.y * G + 2 * lane;
+  const bool live = 2 * lane < G && k2 < K;
+  const int kk = min(k2, K - 2) >> 1;
+  const TapRec* rec = a.tap + m0;
+  const int packed = rec[0].n;
+  const int nx = (packed >> 8) & 15, ny = (packed >> 4) & 15;
+  const int nz = (nx == 0 || ny == 0) ? 0 : (packed & 15);
+  const int p0z = rec[0].p0[0], p0y = rec[0].p0[1], p0x = rec[0].p0[2];
+  const int K2 = K >> 1;
+  int px[5];
+#pragma unroll
+  for (int dx = 0; dx < 5; ++dx) px[dx] = wrap2(p0x + dx, gx);
+  const float4* grid4 = reinterpret_cast<const float4*>(a.grid);
+  float4 acc[kTapGroup];
+#pragma unroll
+  for (int t = 0; t < kTapGroup; ++t) acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int dz = 0; dz < nz; ++dz) {
+    const int pz = wrap2(p0z + dz, gz);
+    float4 v[4][4];
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+      const float4* row = grid4 + (((int64_t)pz * gy + wrap2(p0y + dy, gy)) * gx) * K2;
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) v[dy][dx] = row[(int64_t)px[dx] * K2 + kk];
+    }
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+      if (dy == 4 && ny < 5) break;
+      const float4* row = grid4 + (((int64_t)pz * gy + wrap2(p0y + dy, gy)) * gx) * K2;
+      float4 u[5];
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) u[dx] = dy < 4 ? v[dy][dx] : row[(int64_t)px[dx] * K2 + kk];
+      u[4] = nx == 5 ? row[(int64_t)px[4] * K2 + kk] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int t = 0; t < kTapGroup; ++t) {
+        if (t < gs) {
+          const TapRec& r = rec[t];
+          const float wzy = r.w[dz] * r.w[5 + dy];
+#pragma unroll
+          for (int dx = 0; dx < 4; ++dx) acc4(acc[t], wzy * r.w[10 + dx], u[dx]);
+          if (nx == 5) acc4(acc[t], wzy * r.w[14], u[4]);
+        }
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int t = 0; t < kTapGroup; ++t) {
+    if (t >= gs) break;
+    const int64_t m = m0 + t;
+    const float2 a0 = make_float2(acc[t].x, acc[t].y), a1 = make_float2(acc[t].z, acc[t].w);
+    if (a.mode == kInterpPlain) {
+      const int64_t o = a.perm[m];
+      a.out[(int64_t)k2 * a.M + o] = a0;
+      a.out[(int64_t)(k2 + 1) * a.M + o] = a1;
+    } else {
+      float4* q4 = reinterpret_cast<float4*>(a.q) + m * K2 + (k2 >> 1);
+      if (a.mode == kInterpDual) {
+        const float4 yy = reinterpret_cast<const float4*>(a.y)[m * K2 + (k2 >> 1)];
+        const float4 qo = *q4;
+        const float2 r0 = csub(a0, make_float2(yy.x, yy.y)), r1 = csub(a1, make_float2(yy.z, yy.w));
+        const float t0x = qo.x + a.sigma * r0.x, t0y = qo.y + a.sigma * r0.y;
+        const float t1x = qo.z + a.sigma * r1.x, t1y = qo.w + a.sigma * r1.y;
+        *q4 = make_float4(t0x / a.one_plus_sigma, t0y / a.one_plus_sigma, t1x / a.one_plus_sigma,
+                          t1y / a.one_plus_sigma);
+      } else if (a.mode == kInterpResid) {
+        const float4 yy = reinterpret_cast<const float4*>(a.y)[m * K2 + (k2 >> 1)];
+        *q4 = make_float4(a0.x - yy.x, a0.y - yy.y, a1.x - yy.z, a1.y - yy.w);
+      } else {
+        *q4 = make_float4(a0.x, a0.y, a1.x, a1.y);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic adjoint spreading over clusters of kClZ x kClY x kClX grid
 // points.  A wave owns one cluster and KCI channels (lanes: KCI channels x
 // 64/KCI point slots) and keeps the cluster's sums in registers while it walks
@@ -496,11 +596,14 @@ int launch_interp(const InterpArgs& a, hipStream_t s) {
   const unsigned blocks = grid1d(a.M, 4 * spw);
   if (a.M == 0) return kOk;
   const bool aligned16 = (((uintptr_t)a.grid | (uintptr_t)a.q | (uintptr_t)a.y) & 15) == 0;
-  if (a.kgroups == 2 && K % 2 == 0 && K >= 96 && aligned16) {  // 16-byte loads, groups of <= 128
+  if (a.kgroups >= 2 && K % 2 == 0 && K >= 96 && aligned16) {  // 16-byte loads, groups of <= 128
     const int ng = (K + 127) / 128;
     int G = (K + ng - 1) / ng;
     G += G & 1;
-    hipLaunchKernelGGL(k_interp_v4, dim3(grid1d(a.M, 4), ng), dim3(256), 0, s, a, G);
+    if (a.kgroups == 3 && a.grp && a.ngroups > 0)  // one wave per group of samples with equal taps
+      hipLaunchKernelGGL(k_interp_grp, dim3(grid1d(a.ngroups, 4), ng), dim3(256), 0, s, a, G);
+    else
+      hipLaunchKernelGGL(k_interp_v4, dim3(grid1d(a.M, 4), ng), dim3(256), 0, s, a, G);
     TR_LAUNCH_CHECK();
     return kOk;
   }

@@ -70,6 +70,9 @@ struct BinData {
   int* heavy_row = nullptr;      // heavy cluster ids
   int* heavy_seg_ptr = nullptr;  // nheavy + 1
   int64_t touched = 0;           // distinct grid points under the bin's taps
+  // groups of at most kTapGroup consecutive sorted samples with identical taps
+  int64_t ngroups = 0;
+  int* grp = nullptr;            // ngroups + 1 offsets into the sorted samples
 };
 
 }  // namespace tr
@@ -193,8 +196,12 @@ struct AdjXArgs {
 
 enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2, kInterpResid = 3 };
 
+constexpr int kTapGroup = 4;  // samples sharing one tap gather in the grouped interpolation
+
 struct InterpArgs {
   GridGeom gg;
+  const int* grp;   // tap groups (grouped mode)
+  int64_t ngroups;
   int64_t touched;  // distinct grid points read (algorithmic byte count only)
   int xcd_chunk;
   int kgroups;

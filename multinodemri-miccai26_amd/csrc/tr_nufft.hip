@@ -200,6 +200,8 @@ int tr_nufft_forward(tr_plan* p, int bin, const tr_c64* img, const tr_c64* maps,
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.touched = b.touched;
+  a.grp = b.grp;
+  a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -267,6 +269,8 @@ int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* m
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.touched = b.touched;
+  a.grp = b.grp;
+  a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -292,6 +296,8 @@ int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.touched = b.touched;
+  a.grp = b.grp;
+  a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;
@@ -314,6 +320,8 @@ int tr_data_residual(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   a.kgroups = p->interp_kgroups;
   a.tap = b.tap;
   a.touched = b.touched;
+  a.grp = b.grp;
+  a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;
   a.grid = p->work;

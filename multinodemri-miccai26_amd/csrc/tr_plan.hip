@@ -28,12 +28,27 @@ __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh,
   const float ky = py + sh.y;
   const float kx = px + sh.z;
   kp[m] = make_float4(kz, ky, kx, 0.f);
-  auto tile = [](float k, int g, int n) {
-    int t = (int)floorf((k + 2.0f) / (float)kTile);
-    return t < 0 ? 0 : (t >= n ? n - 1 : t);
-  };
-  const int tz = tile(kz, g.x, nt.x), ty = tile(ky, g.y, nt.y), tx = tile(kx, g.z, nt.z);
-  key[m] = (unsigned)((tz * nt.y + ty) * nt.z + tx);
+  // Key = (kTile^3 tile of the first tap, first tap within the tile, 5-tap flags), so samples
+  // with the same taps (same first tap and tap counts, as axis_taps computes them) are adjacent
+  // in the sorted order and the interpolation can gather their taps once.
+  unsigned tile = 0, local = 0, five = 0;
+  const float kv[3] = {kz, ky, kx};
+  const int ntv[3] = {nt.x, nt.y, nt.z};
+  for (int ax = 0; ax < 3; ++ax) {
+    int t0 = 0, n = 0;
+    if (isfinite(kv[ax]) && fabsf(kv[ax]) <= 1.0e6f) {
+      const double kd = (double)kv[ax];
+      t0 = (int)ceil(kd - 2.0);
+      n = (int)floor(kd + 2.0) - t0 + 1;
+    }
+    int t = (int)floor((t0 + 4) / (double)kTile);
+    t = t < 0 ? 0 : (t >= ntv[ax] ? ntv[ax] - 1 : t);
+    const int off = t0 + 4 - t * kTile;  // in [0, kTile) unless clamped
+    tile = tile * (unsigned)ntv[ax] + (unsigned)t;
+    local = local * (unsigned)(kTile + 2) + (unsigned)(off < 0 ? 0 : (off > kTile ? kTile + 1 : off));
+    five = five * 2u + (n == 5 ? 1u : 0u);
+  }
+  key[m] = (tile * (unsigned)((kTile + 2) * (kTile + 2) * (kTile + 2)) + local) * 8u + five;
   idx[m] = (int)m;
 }
 
@@ -76,6 +91,20 @@ __global__ void k_tap_records(const float4* kp, int64_t M, int3 g, float beta, T
   r.n = packed;
   tap[j] = r;
   cnt[j] = count;
+}
+
+// flag[j] = 1 where sorted sample j's taps differ from sample j - 1's (first tap
+// and tap counts): the starts of runs of samples with identical taps.
+__global__ void k_tap_run_flags(const TapRec* tap, int64_t M, unsigned char* flag) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  bool start = j == 0;
+  if (!start) {
+    const TapRec& a = tap[j];
+    const TapRec& b = tap[j - 1];
+    start = a.n != b.n || a.p0[0] != b.p0[0] || a.p0[1] != b.p0[1] || a.p0[2] != b.p0[2];
+  }
+  flag[j] = start ? 1 : 0;
 }
 
 // Distinct clusters touched along one axis, with the weight of every cluster
@@ -271,7 +300,9 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
                      kp_raw, key_in, idx);
   TR_LAUNCH_CHECK();
   int key_bits = 1;
-  while (((int64_t)1 << key_bits) < (int64_t)nt.x * nt.y * nt.z) ++key_bits;
+  while (((int64_t)1 << key_bits) < (int64_t)nt.x * nt.y * nt.z * (kTile + 2) * (kTile + 2) * (kTile + 2) * 8)
+    ++key_bits;
+  if (key_bits > 32) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
   size_t tmp_bytes = 0;
   TR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key_in, key_out, idx, b.perm, (int)M, 0,
                                             key_bits, s));
@@ -289,6 +320,29 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   TR_HIP(hipMemsetAsync(cnt + M, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_tap_records, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, g3, p->beta, b.tap, cnt);
   TR_LAUNCH_CHECK();
+  {  // groups of up to kTapGroup consecutive samples with identical taps
+    unsigned char* flag = nullptr;
+    TR_HIP(hipMalloc(&flag, (size_t)M));
+    hipLaunchKernelGGL(k_tap_run_flags, dim3(grid1d(M, 256)), dim3(256), 0, s, b.tap, M, flag);
+    std::vector<unsigned char> hf((size_t)M);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hf.data(), flag, (size_t)M, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(flag);
+    TR_HIP(e);
+    std::vector<int> starts;
+    starts.reserve((size_t)M / 2 + 2);
+    int run = 0;
+    for (int64_t j = 0; j < M; ++j) {
+      if (hf[(size_t)j] || run == kTapGroup) run = 0;
+      if (run == 0) starts.push_back((int)j);
+      ++run;
+    }
+    starts.push_back((int)M);
+    b.ngroups = (int64_t)starts.size() - 1;
+    TR_HIP(hipMalloc(&b.grp, sizeof(int) * starts.size()));
+    TR_HIP(hipMemcpy(b.grp, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice));
+  }
   size_t scan_bytes = 0;
   TR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, (int)(M + 1), s));
   void* scan_tmp = nullptr;
@@ -451,6 +505,7 @@ void free_plan(tr_plan* p) {
     (void)hipFree(b.perm);
     (void)hipFree(b.kp);
     (void)hipFree(b.tap);
+    (void)hipFree(b.grp);
     (void)hipFree(b.row_ptr);
     (void)hipFree(b.pairs);
     (void)hipFree(b.seg_row);
@@ -618,7 +673,7 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
       p->prune = (int)value;
       return kOk;
     case TR_PARAM_INTERP_KGROUPS:
-      if (value < 0 || value > 2) TR_FAIL(kInvalidArgument, "interp channel groups is 0, 1 or 2");
+      if (value < 0 || value > 3) TR_FAIL(kInvalidArgument, "interp channel groups is 0 .. 3");
       p->interp_kgroups = (int)value;
       return kOk;
     case TR_PARAM_WORKSPACE_SLOTS:

@@ -241,11 +241,19 @@ class HipPdhgBase:
         gen = torch.Generator(device=self.device)
         gen.manual_seed(int(seed))
         mx = 0.0
+        K = self.E * self.C
         for b in range(self.B):
-            v = torch.view_as_real(self.y[b])
-            v.add_(torch.randn(v.shape, generator=gen, device=self.device, dtype=torch.float32), alpha=sig)
-            if self.y[b].numel():
+            M = self.y[b].shape[0]
+            # noise drawn per (channel, sample) in trajectory order and moved to the sorted layout,
+            # so the realisation does not depend on the plan's internal sample order
+            raw = torch.randn((K, M, 2), generator=gen, device=self.device, dtype=torch.float32)
+            noise = torch.empty_like(self.y[b])
+            if M:
+                _lib.call("tr_plan_gather_samples", self.plan.handle, b, torch.view_as_complex(raw).data_ptr(), K,
+                          noise.data_ptr(), s)
+                self.y[b].add_(noise, alpha=sig)
                 mx = max(mx, float(torch.max(torch.abs(self.y[b]))))
+            del raw, noise
         if normalize and mx > 0:
             for b in range(self.B):
                 self.y[b].div_(np.float32(mx))

@@ -21,7 +21,9 @@ import os
 from . import _lib
 
 INTERP_CANDIDATES = (0, 16, 64, 256, 1024)
-INTERP_KGROUPS = (0, 1, 2)  # all channels per block / 64-channel groups / 16-byte loads, <=128-channel groups
+# all channels per block / 64-channel groups / 16-byte loads over <=128-channel groups / the same with
+# one gather per group of samples with identical taps
+INTERP_KGROUPS = (0, 1, 2, 3)
 SPREAD_CANDIDATES = (0, 64)
 
 _cache: dict = {}

@@ -116,11 +116,12 @@ def test_launch_params_and_autotune_keep_results_bitwise(torch_mod):
 @pytest.mark.gpu
 def test_interp_channel_group_modes_bitwise(torch_mod):
     """96 channels (16 coils x 6 echoes): the interpolation's channel-group modes (all channels
-    per block, 64-channel groups, 16-byte loads over 96-channel groups) give the same bits."""
-    acq, ksp, prob = make_problem(shape=(10, 12, 14), coils=16, echoes=6, bins=2, seed=13)
+    per block, 64-channel groups, 16-byte loads over 96-channel groups, and the same with one tap
+    gather per group of samples with identical taps) give the same bits."""
+    acq, ksp, prob = make_problem(shape=(10, 12, 14), coils=16, echoes=6, bins=2, interleaves=400, seed=13)
     s = _hip_solver("tvme", acq, ksp, 3, 0.0, True)
     outs = []
-    for kg in (0, 1, 2):
+    for kg in (0, 1, 2, 3):
         s.plan.set_param("interp_kgroups", kg)
         outs.append(s.run().cpu().numpy())
     for o in outs[1:]:
