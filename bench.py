@@ -65,7 +65,8 @@ def parse_args(argv=None):
                          "WRITE_SIZE) of a run with --copy-calibration-mib into profiles/traffic.json")
     ap.add_argument("--params", default=None,
                     help="JSON launch parameters (interp_xcd_chunk, interp_kgroups, spread_xcd_chunk, "
-                         "bin_streams) to set instead of autotuning (profiling runs)")
+                         "lines_xcd_lines, bin_streams) to set instead of autotuning "
+                         "(profiling runs)")
     ap.add_argument("--legs", choices=("all", "plain"), default="all",
                     help="plain: only the plain PDHG leg (no warm start, no power iteration)")
     ap.add_argument("--cg-steps", type=int, default=None,
@@ -398,7 +399,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     if args.params:
         tuned = json.loads(args.params)
-        for k in ("interp_xcd_chunk", "interp_kgroups", "spread_xcd_chunk"):
+        for k in ("interp_xcd_chunk", "interp_kgroups", "spread_xcd_chunk", "lines_xcd_lines"):
             if k in tuned:
                 solver.plan.set_param(k, int(tuned[k]))
         solver.set_bin_streams(int(tuned.get("bin_streams", 1)))

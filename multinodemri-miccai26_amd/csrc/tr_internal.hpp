@@ -41,7 +41,7 @@ struct __align__(16) TapRec {
 // The adjoint gathers over clusters of kClZ x kClY x kClX grid points: a wave
 // keeps the cluster's sums for 64 channels in registers and walks the sorted
 // samples touching the cluster once.
-constexpr int kClZ = 2, kClY = 4, kClX = 4;
+constexpr int kClZ = 4, kClY = 4, kClX = 2;
 constexpr int kClPts = kClZ * kClY * kClX;
 
 // One (cluster, sample) pair: the sample's sorted index and its weights on the
