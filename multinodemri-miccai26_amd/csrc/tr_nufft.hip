@@ -14,7 +14,6 @@ namespace tr {
 static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, int E, int C,
                            hipStream_t s, bool prune = true) {
   prune = prune && p->prune && p->col_win;
-  prune = prune && p->prune && p->col_win;
   const int K = E * C;
   int r = ensure_work(p, K);
   if (r) return r;
@@ -37,7 +36,10 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     const int kc = fft_kc(C);
     dim3 grid((unsigned)(nz * ny * ((C + kc - 1) / kc)));
     ProfScope ps(kKFwdXPad, 8.0 * (E + C) * p->N + 8.0 * K * nz * ny * gx, s);
-    if (int r = launch_fwd_x(kc, gx, grid, fft_smem(gx, kc), s, a)) return r;
+    // LDS: twiddles + FFT tile + the E image rows of the block (staged once for its coils)
+    const size_t smem = fft_smem(gx, kc) + sizeof(float2) * (size_t)E * p->n[2];
+    if (smem > 65536) TR_FAIL(kUnsupported, "too many echoes per forward launch (LDS)");
+    if (int r = launch_fwd_x(kc, gx, grid, smem, s, a)) return r;
   }
   const int kc = fft_kc(K);
   const int nchunk = (K + kc - 1) / kc;

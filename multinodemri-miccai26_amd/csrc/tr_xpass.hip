@@ -39,15 +39,18 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
     constexpr int N1 = split_n1(LC), N2 = LC / N1;
     // Every load below reads a clamped, in-bounds address and the padding is applied with a
     // select afterwards, so no load sits in a per-lane branch and all of them are in flight at once.
+    // The E image rows (shared by the block's 16 coils) are staged in LDS behind the FFT tile.
+    const int E = a.K / a.C;
+    float2* img_row = buf + LC * T::ld;
+    for (int i = tid; i < E * nx; i += kBlock) {
+      const int e = i / nx;
+      img_row[i] = a.img[(int64_t)e * N + rowvox + (i - e * nx)];
+    }
     const int cc = min(c, a.C - 1);
-    float2 sm[N1], u[N1];
+    float2 sm[N1];
     float ap[N1];
 #pragma unroll
-    for (int r = 0; r < N1; ++r) {
-      const int xc = min(pmod(w + N2 * r + nx / 2, gx), nx - 1);
-      ap[r] = a.apod_x[xc];
-      u[r] = a.img[rowvox + xc];
-    }
+    for (int r = 0; r < N1; ++r) ap[r] = a.apod_x[min(pmod(w + N2 * r + nx / 2, gx), nx - 1)];
     if (a.maps) {
 #pragma unroll
       for (int r = 0; r < N1; ++r) sm[r] = a.maps[(int64_t)cc * N + rowvox + min(pmod(w + N2 * r + nx / 2, gx), nx - 1)];
@@ -66,17 +69,13 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
       sm[r] = (w < N2 && x < nx && live) ? cscale(sm[r], azy * ap[r]) : make_float2(0.f, 0.f);
     }
     const int zp = pmod(z - nz / 2, gz), yp = pmod(y - ny / 2, gy);
-    const int E = a.K / a.C;
+    __syncthreads();  // image rows staged
     for (int e = 0; e < E; ++e) {
       float2 v[N1];
       // sm is zero at padding positions, so the product is the zero-padded, coil-weighted row
 #pragma unroll
-      for (int r = 0; r < N1; ++r) v[r] = cmul(u[r], sm[r]);
-      if (e + 1 < E) {
-#pragma unroll
-        for (int r = 0; r < N1; ++r)
-          u[r] = a.img[(int64_t)(e + 1) * N + rowvox + min(pmod(w + N2 * r + nx / 2, gx), nx - 1)];
-      }
+      for (int r = 0; r < N1; ++r)
+        v[r] = cmul(img_row[e * nx + min(pmod(w + N2 * r + nx / 2, gx), nx - 1)], sm[r]);
       __syncthreads();  // twiddles / the previous echo's pass-2 reads
       if (w < N2) two_level_pass1<LC>(v, buf, T::ld, ch, w, tw);
       __syncthreads();
