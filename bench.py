@@ -10,13 +10,13 @@ full solve; inputs are resident in HBM when the timed region starts.
 
 Two legs run on the same problem: plain PDHG from zeros (as the reference),
 and the learned warm start (density-compensated adjoint -> 3-D CNN on MFMA ->
-per-(bin, echo) data-fitted scale -> 3 conjugate-gradient data-consistency
+per-(bin, echo) data-fitted scale -> 4 conjugate-gradient data-consistency
 steps -> PDHG from there to the same tolerance, the warm-start work inside the
 timed region).  ``value`` is the learned leg's time-to-tol when its final
 objective is within +-0.1 % of the plain leg's, otherwise the plain leg's (the
 fallback); both legs are reported.  Variant starts (CNN without CG, CG from
-zeros without the CNN, CNN with 5 CG steps) are timed too and reported with
-their signed objective differences; they never set ``value``.
+zeros without the CNN, CNN with 3 or 5 CG steps) are timed too and reported
+with their signed objective differences; they never set ``value``.
 
 Prints one JSON line: time-to-tol (value, seconds), iterations, per-kernel
 algorithmic GB/s from HIP events recorded on the launch stream (plain leg),
@@ -70,8 +70,8 @@ def parse_args(argv=None):
     ap.add_argument("--legs", choices=("all", "plain"), default="all",
                     help="plain: only the plain PDHG leg (no warm start, no power iteration)")
     ap.add_argument("--cg-steps", type=int, default=None,
-                    help="CG steps of the learned leg (default: the warm-start module's default, 3)")
-    ap.add_argument("--variants", default="cnn+0,zero+3,cnn+5",
+                    help="CG steps of the learned leg (default: the warm-start module's default, 4)")
+    ap.add_argument("--variants", default="cnn+0,zero+4,cnn+3,cnn+5",
                     help="comma list of extra warm starts 'cnn+K' / 'zero+K' (K CG steps) timed and "
                          "reported beside the learned leg; '' for none")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

@@ -16,11 +16,13 @@ data normal equations A^H W A u = A^H W y from u0 (float64 step sizes and
 residual sums on the GPU): CG removes the well-sampled part of the remaining
 error much faster than PDHG's fixed steps do.  ``warm_start`` chains the two.
 
-DEFAULT_CG_STEPS = 3 was chosen on held-out problems of the training size
-(64x96x96, 16 coils, 4 echoes, 6 bins, seeds 9000-9002): it is the largest
-step count whose final objective stayed within +-0.1 % of the plain solve on
-every seed (3 steps: 1.9-2.3x faster to tol 1e-3, +3e-4 to +5e-4; 5 steps:
-4-5x faster, but up to 0.15 % *below* the plain objective).
+DEFAULT_CG_STEPS = 4 was chosen on held-out problems: the training size
+(64x96x96, 16 coils, 4 echoes, 6 bins, seeds 9000-9003) and the full size
+(80x192x192, 32 coils, 6 echoes, 6 bins, generator seeds 2 and 3).  It is
+the largest step count whose final objective stayed within +-0.1 % of the
+plain solve on every one of them (4 steps: 2.5-3.3x faster to tol 1e-3 at
+the training size, +4e-4 to +8e-4; 3.9x at full size, -1.4e-4; 5 steps ends
+up to 0.14 % *below* the plain objective).
 """
 
 from __future__ import annotations
@@ -31,7 +33,7 @@ from .. import _lib
 
 FIT_BLOCKS = 512
 CG_BLOCKS = 1024
-DEFAULT_CG_STEPS = 3
+DEFAULT_CG_STEPS = 4
 
 
 def dc_adjoint(solver, out=None):

@@ -61,7 +61,7 @@ def build_parser(algorithm: str) -> argparse.ArgumentParser:
         p.add_argument("--warm-start", choices=("none", "learned"), default="none")
         p.add_argument("--warm-start-weights", type=str, default=None)
         p.add_argument("--warm-start-cg-steps", type=int, default=None,
-                       help="conjugate-gradient data-consistency steps after the CNN (default 3)")
+                       help="conjugate-gradient data-consistency steps after the CNN (default 4)")
     p.add_argument("--prepared-dir", type=str, default=None)
     p.add_argument("--output-dir", type=str, default=".")
     p.add_argument("--verbose", action="store_true")

@@ -91,5 +91,5 @@ def test_tvme_cli_learned_warm_start(torch_mod, tmp_path):
     assert abs(phi_w - phi_p) / phi_p < 3e-2
     man = json.loads((tmp_path / "warm" / "run_manifest.json").read_text())
     assert man["parameters"]["warm_start"] == "learned"
-    assert man["parameters"]["warm_start_cg_steps"] == 3
+    assert man["parameters"]["warm_start_cg_steps"] == 4
     assert "warm_start" not in json.loads((tmp_path / "plain" / "run_manifest.json").read_text())["parameters"]

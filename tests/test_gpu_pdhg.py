@@ -155,14 +155,14 @@ def test_learned_warm_start_gains_over_plain(torch_mod):
     phi_warm = objective.objective(solver, u_warm)
     assert it_warm <= 0.85 * it_plain, (it_warm, it_plain)
     assert abs(phi_warm - phi_plain) / phi_plain < 1e-3
-    # CNN followed by the default conjugate-gradient data-consistency steps: about half the
-    # iterations (0.40-0.48 of plain on seeds 9000-9002), the same objective within 0.1 %
+    # CNN followed by the default conjugate-gradient data-consistency steps: about a third of the
+    # iterations (0.26-0.34 of plain on seeds 9000-9002), the same objective within 0.1 %
     from toporecon_hip.learned import warm_start
     u_cg = solver.run(init=warm_start(solver, net), dual_init="residual")
     it_cg = solver.iterations
     phi_cg = objective.objective(solver, u_cg)
     print("plain", it_plain, "cnn", it_warm, "cnn+cg", it_cg, "objective", phi_plain, phi_warm, phi_cg, flush=True)
-    assert it_cg <= 0.55 * it_plain, (it_cg, it_plain)
+    assert it_cg <= 0.45 * it_plain, (it_cg, it_plain)
     assert abs(phi_cg - phi_plain) / phi_plain < 1e-3
 
 
