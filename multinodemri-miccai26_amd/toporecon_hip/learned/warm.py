@@ -98,8 +98,9 @@ def fit_scales(solver, v):
     return c
 
 
-def cnn_warm_start(solver, net, chunk=6):
-    """u0 (B, E, Z, Y, X) complex64 for ``solver.run(init=u0, dual_init="residual")``."""
+def cnn_warm_start(solver, net, chunk=6, keep_rhs=False):
+    """u0 (B, E, Z, Y, X) complex64 for ``solver.run(init=u0, dual_init="residual")``; with
+    keep_rhs, (u0, A^H W y) -- the network's input, which ``cg_refine`` can reuse as its rhs."""
     import torch
     shape = tuple(solver.img_shape)
     B, E = solver.B, solver.E
@@ -107,12 +108,14 @@ def cnn_warm_start(solver, net, chunk=6):
     g = dc_adjoint(solver)
     imgs = g.reshape(B * E, *shape)
     v, x, out, scale = predict(net, imgs, shape, chunk=chunk)
-    del g
+    if not keep_rhs:
+        del g
     c = fit_scales(solver, v.reshape(B, E, *shape))
     scale2 = scale * torch.from_numpy(c.reshape(-1).astype(np.float32)).to(scale.device)
     _lib.call("tr_cnn_unpack", x.data_ptr(), out.data_ptr(), scale2.data_ptr(), V, B * E, v.data_ptr(),
               net.stream)
-    return v.reshape(B, E, *shape)
+    u0 = v.reshape(B, E, *shape)
+    return (u0, g) if keep_rhs else u0
 
 
 def _host_sum(partial):
@@ -129,8 +132,11 @@ def _normal(solver, x, out, qtmp):
                   solver.maps.data_ptr(), solver.E, solver.C, out[b].data_ptr(), 0, s)
 
 
-def cg_refine(solver, x0, steps: int):
-    """`steps` conjugate-gradient iterations on A^H W A u = A^H W y from x0 (returns a new array)."""
+def cg_refine(solver, x0, steps: int, rhs=None, x0_zero: bool = False):
+    """`steps` conjugate-gradient iterations on A^H W A u = A^H W y from x0 (returns a new array).
+
+    rhs: A^H W y if the caller has it (the buffer is consumed as the residual); x0_zero: x0 is
+    known to be zero, so the initial residual is the rhs itself."""
     import torch
     if steps <= 0:
         return x0
@@ -139,15 +145,18 @@ def cg_refine(solver, x0, steps: int):
     dev = x0.device
     x = torch.empty_like(x0)
     _lib.call("tr_cscale", x0.data_ptr(), n, 1.0, x.data_ptr(), s)
-    r = dc_adjoint(solver)
+    r = rhs if rhs is not None else dc_adjoint(solver)
     qtmp = [torch.empty_like(solver.y[b]) for b in range(solver.B)]
     ap = torch.empty_like(x)
     p = torch.empty_like(x)
     partial = torch.empty(CG_BLOCKS, dtype=torch.float64, device=dev)
-    _normal(solver, x, ap, qtmp)
-    # r = A^H W y - A^H W A x (p is scratch for the x-side update here)
-    _lib.call("tr_cg_update", p.data_ptr(), r.data_ptr(), ap.data_ptr(), ap.data_ptr(), n, 1.0,
-              partial.data_ptr(), CG_BLOCKS, s)
+    if x0_zero:
+        _lib.call("tr_cdot_re", r.data_ptr(), r.data_ptr(), n, partial.data_ptr(), CG_BLOCKS, s)
+    else:
+        _normal(solver, x, ap, qtmp)
+        # r = A^H W y - A^H W A x (p is scratch for the x-side update here)
+        _lib.call("tr_cg_update", p.data_ptr(), r.data_ptr(), ap.data_ptr(), ap.data_ptr(), n, 1.0,
+                  partial.data_ptr(), CG_BLOCKS, s)
     rr = _host_sum(partial)
     _lib.call("tr_cscale", r.data_ptr(), n, 1.0, p.data_ptr(), s)
     for _ in range(steps):
@@ -172,7 +181,9 @@ def warm_start(solver, net=None, cg_steps: int = DEFAULT_CG_STEPS, chunk=6):
     """Warm start u0: the network's estimate (zeros without a network), then cg_steps CG steps."""
     import torch
     if net is not None:
-        u0 = cnn_warm_start(solver, net, chunk=chunk)
-    else:
-        u0 = torch.zeros(solver.vol_dims(), dtype=torch.complex64, device=solver.device)
-    return cg_refine(solver, u0, cg_steps)
+        if cg_steps <= 0:
+            return cnn_warm_start(solver, net, chunk=chunk)
+        u0, rhs = cnn_warm_start(solver, net, chunk=chunk, keep_rhs=True)
+        return cg_refine(solver, u0, cg_steps, rhs=rhs)
+    u0 = torch.zeros(solver.vol_dims(), dtype=torch.complex64, device=solver.device)
+    return cg_refine(solver, u0, cg_steps, x0_zero=True)

@@ -96,3 +96,26 @@ def test_cg_refine_matches_float64_cg(torch_mod, start):
     assert all(b < a for a, b in zip(data, data[1:])), data
     assert rel(x_gpu, x_ref) < 1e-4
     assert cg_refine(s, x0, 0) is x0
+
+
+def test_warm_start_reuses_rhs_and_zero_start(torch_mod):
+    """warm_start's shortcuts (the CNN's input reused as the CG rhs; no normal-operator product for
+    a zero start) give the same result as the plain composition."""
+    torch = torch_mod
+    from toporecon_hip.algorithms import TvmeReconstructor
+    from toporecon_hip.learned import WarmStartNet, cg_refine, cnn_warm_start, default_weights_path, warm_start
+    acq, ksp, prob = make_problem(shape=(12, 16, 16), coils=4, echoes=2, bins=2, interleaves=200,
+                                  readouts=40, seed=9, algorithm="tvme")
+    B = len(acq.bin_edges) - 1
+    s = TvmeReconstructor(ksp, acq.coord, acq.dcf, acq.mps, acq.resp, np.zeros_like(ksp), B, True,
+                          lamda_m=2e-3, lamda_e=2e-3, sigma=1 / 6, tau=1 / 6, max_iter=1, tol=0.0, margin=0,
+                          device=0, bin_edges=acq.bin_edges, show_pbar=False)
+    s.pdhg_init()
+    net = WarmStartNet.load(default_weights_path(), 0)
+    a = warm_start(s, net, cg_steps=3).cpu().numpy()
+    b = cg_refine(s, cnn_warm_start(s, net), 3).cpu().numpy()
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    z = torch.zeros(s.vol_dims(), dtype=torch.complex64, device="cuda")
+    c = warm_start(s, None, cg_steps=3).cpu().numpy()
+    d = cg_refine(s, z, 3).cpu().numpy()
+    assert rel(c, d) < 1e-6
