@@ -277,7 +277,7 @@ def bench_batched(args, verbose=True):
                              margin=0, device=dev, bin_edges=acq.bin_edges, show_pbar=False,
                              echoes=len(cfg.echo_times_ms))
         s.simulate_kspace(acq.truth, cfg.snr_db, seed=cfg.seed + 104729)
-        tuned = s.autotune(reps=1)
+        tuned = s.autotune(reps=1, bin_streams=False)  # graph batches keep one stream per instance
         solvers.append(s)
         rows.append({"seed": cfg.seed, "shape": list(cfg.img_shape), "coils": cfg.coils,
                      "echoes": len(cfg.echo_times_ms), "bins": cfg.bins,

@@ -90,6 +90,9 @@ class GraphBatch:
         import torch
         if mode not in ("graph", "streams"):
             raise ValueError(f"unknown mode {mode!r}")
+        if mode == "graph" and any(getattr(s, "_bstreams", None) for s in self.solvers):
+            # per-bin streams forked inside a capture are not supported: set_bin_streams(1) first
+            raise ValueError("graph mode needs every instance on one bin stream (set_bin_streams(1))")
         self._prepare()
         n = len(self.solvers)
         its = [0] * n

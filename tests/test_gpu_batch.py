@@ -87,3 +87,14 @@ def test_graph_batch_stops_each_instance_on_its_rule(torch_mod):
     for s, (u, n), o in zip(solvers, ref, out):
         assert o["iterations"] == n
         assert torch.equal(s.ubar, u)
+
+
+def test_graph_mode_refuses_per_bin_streams(torch_mod):
+    """Graph batches run each instance's bins on one stream: an instance set to several bin
+    streams is refused before anything is captured."""
+    from toporecon_hip.batch import GraphBatch
+    acq, ksp, _ = make_problem(shape=(8, 10, 12), bins=3, seed=35)
+    s = _solver("tvm", acq, ksp, 2, 0.0)
+    s.set_bin_streams(2)
+    with pytest.raises(ValueError):
+        GraphBatch([s]).run(mode="graph")
