@@ -522,6 +522,20 @@ def main(argv=None):
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["gbps"], "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": dk["gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
                 "algorithmic_bytes_per_launch": dk["bytes_per_launch"], "avg_launch_ms": dk["ms_avg"]}
+    # the two gathers take about the same time at this size, so the dominant one can flip between
+    # runs: both are reported
+    gathers = {}
+    all_traffic = {}
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            all_traffic = json.load(f).get("kernels", {})
+    for k in ("interp", "spread"):
+        if k in kernels and kernels[k]["gbps"]:
+            gathers[k] = {"achieved": kernels[k]["gbps"], "frac": kernels[k]["gbps"] / HBM_PEAK_GBPS,
+                          "avg_launch_ms": kernels[k]["ms_avg"],
+                          "algorithmic_bytes_per_launch": kernels[k]["bytes_per_launch"],
+                          "traffic": all_traffic.get(k, {}).get("bytes")}
+    roofline["gathers"] = gathers
     fwd_b, adj_b = survey_nufft_bytes(samples, cfg.echoes, cfg.coils, cfg.img_shape,
                                       solver.plan.grid_shape)
     fwd_ms = sum(kernels[k]["ms_total"] for k in ("fwd_x_pad", "fft_y_fwd", "fft_z_fwd", "interp")
