@@ -96,8 +96,9 @@ def make_config(name):
     return cfg
 
 
-def survey_nufft_bytes(plan_samples, E, C, img_shape, grid_shape):
-    """Algorithmic bytes of one forward and one adjoint pass over all bins.
+def nufft_model_bytes(plan_samples, E, C, img_shape, grid_shape):
+    """Algorithmic bytes of one forward and one adjoint pass over all bins, on the unpruned model
+    (every grid pass moves the whole oversampled grid), for the NUFFT GB/s figures:
 
     forward per bin: pad 8EN + 8CN + 8KG, fft3 16KG, interp 8KG + 12M + 24KM;
     adjoint per bin: spread 8KM + 16M + 8KG, ifft3 16KG, crop 8KN + 8CN + 8EN.
@@ -536,7 +537,7 @@ def main(argv=None):
                           "algorithmic_bytes_per_launch": kernels[k]["bytes_per_launch"],
                           "traffic": all_traffic.get(k, {}).get("bytes")}
     roofline["gathers"] = gathers
-    fwd_b, adj_b = survey_nufft_bytes(samples, cfg.echoes, cfg.coils, cfg.img_shape,
+    fwd_b, adj_b = nufft_model_bytes(samples, cfg.echoes, cfg.coils, cfg.img_shape,
                                       solver.plan.grid_shape)
     fwd_ms = sum(kernels[k]["ms_total"] for k in ("fwd_x_pad", "fft_y_fwd", "fft_z_fwd", "interp")
                  if k in kernels)
