@@ -52,9 +52,10 @@ def parse_args(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=("full", "small", "batched"), default="full",
+    ap.add_argument("--config", choices=("full", "small", "batched", "tvm48"), default="full",
                     help="full: TVME 80x192x192 (the headline); small: a reduced TVME; batched: 12 uneven "
-                         "TVM instances, HIP-graph batch vs sequential solves")
+                         "TVM instances, HIP-graph batch vs sequential solves; tvm48: plain TVM at "
+                         "48x64x64, 8 coils, 2 echoes, 4 bins beside one full float64 oracle iteration")
     ap.add_argument("--max-iter", type=int, default=None, help="override max_iter (profiling runs)")
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -340,8 +341,96 @@ def bench_batched(args, verbose=True):
     return 0
 
 
+def tvm48_oracle_iteration():
+    """One full float64-oracle TVM PDHG iteration at 48x64x64 (all bins, coils, echoes), 1 thread."""
+    import numpy as np
+
+    from oracle import nufft as on
+    from oracle import pdhg as op
+    from toporecon_hip import synth
+    cfg = synth.config_small_tvm()
+    acq = synth.make_acquisition(cfg)
+    ksp = synth.simulate_kspace_numpy(acq, lambda x, c: on.nufft(x.astype(np.complex128), c))
+    prob = op.bin_problem(ksp, acq.coord, acq.dcf, acq.mps, acq.resp, acq.bin_edges)
+    t0 = time.perf_counter()
+    op.run(prob, "tvm", lam_m=cfg.lambda_motion, l2_coupling=cfg.l2_coupling, max_iter=1, tol=0.0)
+    print(json.dumps({"per_iter": time.perf_counter() - t0, "bins": prob.B, "K": prob.E * prob.C,
+                      "samples": [int(k.shape[-1]) for k in prob.ksp]}))
+
+
+def bench_tvm48(args, verbose=True):
+    """Plain TVM to tol 1e-3 at 48x64x64 (8 coils, 2 echoes, 4 bins, 1536x256 cone samples) on the GPU,
+    beside one full float64-oracle iteration on one host core, scaled to the GPU's iteration count."""
+    import torch
+
+    from toporecon_hip import synth
+    from toporecon_hip.algorithms import TvmReconstructor
+    dev = torch.device("cuda", 0)
+    cfg = synth.config_small_tvm()
+    acq = synth.make_acquisition(cfg)
+    s = TvmReconstructor(None, acq.coord, acq.dcf, acq.mps, acq.resp, None, cfg.bins, cfg.l2_coupling,
+                         lamda=cfg.lambda_motion, sigma=1 / 6, tau=1 / 6, max_iter=cfg.max_iter, tol=cfg.tol,
+                         margin=0, device=dev, bin_edges=acq.bin_edges, show_pbar=False,
+                         echoes=len(cfg.echo_times_ms))
+    s.simulate_kspace(acq.truth, cfg.snr_db, seed=cfg.seed + 104729)
+    tuned = s.autotune(reps=2)
+    for _ in range(max(args.warmup, 1)):
+        s.max_iter, full = 3, s.max_iter
+        s.run()
+        s.max_iter = full
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.run()
+    torch.cuda.synchronize(dev)
+    t = (time.perf_counter() - t0) / args.steps
+    n_it = s.iterations
+    log(f"tvm48: {n_it} iterations, {t * 1e3:.1f} ms to tol ({t / n_it * 1e3:.2f} ms per iteration)", verbose)
+    cpu = None
+    if not args.no_cpu_baseline:
+        env = dict(os.environ)
+        for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+            env[k] = "1"
+        log("cpu baseline: one float64 oracle iteration on 1 core", verbose)
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", "--config", "tvm48"],
+                           capture_output=True, text=True, env=env, timeout=900)
+        if r.returncode == 0:
+            d = json.loads(r.stdout.strip().splitlines()[-1])
+            cpu = {"value": d["per_iter"] * n_it, "unit": "s", "cores": 1, "kind": "port",
+                   "s_per_iteration": d["per_iter"],
+                   "sample": f"one full float64-oracle TVM iteration ({d['bins']} bins x {d['K']} channels, "
+                             f"{sum(d['samples'])} samples) = {d['per_iter']:.1f} s on 1 thread, "
+                             f"x{n_it} iterations"}
+        else:
+            log(f"cpu baseline failed: {r.stderr[-2000:]}", True)
+    line = {
+        "metric": "TVM time-to-tol 1e-3 at 48x64x64, 8 coils, 2 echoes, 4 bins on 1 GPU",
+        "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic: seeded cone trajectory, ellipsoid phantom, Gaussian-blob coil maps, "
+                                "k-space simulated by the HIP forward NUFFT + 30 dB complex noise",
+        "config": {"workload": f"plain TVM PDHG to tol {cfg.tol} (max {cfg.max_iter} it), 48x64x64, "
+                               f"{cfg.coils} coils, 2 echoes, {cfg.bins} bins, 1536x256 cone samples/echo",
+                   "iterations": n_it, "s_per_iteration": t / max(n_it, 1),
+                   "final_tol": s.history[-1] if s.history else None,
+                   "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "interp_kgroups",
+                                                                "spread_xcd_chunk", "bin_streams")}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main(argv=None):
     args = parse_args(argv)
+    if args.cpu_baseline_worker and args.config == "tvm48":
+        tvm48_oracle_iteration()
+        return 0
+    if args.config == "tvm48":
+        if args.gpus != 1:
+            print("bench.py measures one GPU (run with --gpus 1)", file=sys.stderr)
+            return 2
+        return bench_tvm48(args)
     if args.config == "batched":
         if args.gpus != 1:
             print("bench.py measures one GPU (run with --gpus 1)", file=sys.stderr)
