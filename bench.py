@@ -52,10 +52,12 @@ def parse_args(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=("full", "small", "batched", "tvm48"), default="full",
+    ap.add_argument("--config", choices=("full", "small", "batched", "tvm48", "cnn"), default="full",
                     help="full: TVME 80x192x192 (the headline); small: a reduced TVME; batched: 12 uneven "
                          "TVM instances, HIP-graph batch vs sequential solves; tvm48: plain TVM at "
-                         "48x64x64, 8 coils, 2 echoes, 4 bins beside one full float64 oracle iteration")
+                         "48x64x64, 8 coils, 2 echoes, 4 bins beside one full float64 oracle iteration; cnn: "
+                         "the warm-start network's MFMA convolutions (inference at 80x192x192, training "
+                         "steps at 64x96x96)")
     ap.add_argument("--max-iter", type=int, default=None, help="override max_iter (profiling runs)")
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -421,8 +423,87 @@ def bench_tvm48(args, verbose=True):
     return 0
 
 
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def bench_cnn(args, verbose=True):
+    """The warm-start CNN on MFMA: inference over the 36 (bin, echo) images of an 80x192x192 TVME problem
+    (chunks of 6, as the warm start runs it) and full training steps (forward, loss, backward, Adam) on
+    64x96x96 images; executed bf16 MFMA flops over wall time between device synchronisations."""
+    import torch
+
+    from toporecon_hip.learned import WarmStartNet, default_weights_path
+    from toporecon_hip.learned.net import CIN
+    dev = torch.device("cuda", 0)
+    net = WarmStartNet.load(default_weights_path(), dev)
+    flop_voxel = sum(2 * 27 * cin * 32 for cin in CIN)  # five 3x3x3 layers as executed (32 output lanes)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3)
+
+    def rand_input(n, shape):
+        x = torch.zeros((n, *shape, 8), dtype=torch.bfloat16, device=dev)
+        x[..., :2] = torch.randn((n, *shape, 2), generator=gen, device=dev).to(torch.bfloat16)
+        return x
+
+    shape, n, chunk = (80, 192, 192), 36, 6
+    x = rand_input(n, shape)
+    for _ in range(max(args.warmup, 1)):
+        net.forward(x[:chunk], chunk, shape)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for i0 in range(0, n, chunk):
+            net.forward(x[i0:i0 + chunk], chunk, shape)
+    torch.cuda.synchronize(dev)
+    t_inf = (time.perf_counter() - t0) / args.steps
+    vox = n * shape[0] * shape[1] * shape[2]
+    inf_tflops = flop_voxel * vox / t_inf / 1e12
+    log(f"cnn inference: {n} images of {shape} in {t_inf * 1e3:.1f} ms, {inf_tflops:.0f} TFLOP/s", verbose)
+    del x
+    tshape, steps = (64, 96, 96), 20
+    xt = rand_input(1, tshape)
+    target = torch.randn((1, *tshape, 2), generator=gen, device=dev)
+    # a training step: forward (4 bf16 layers + fp32 output) + input-gradient convolutions of layers
+    # 4..1 + weight gradients of all five layers, each the same contraction size as a forward layer
+    flop_step_voxel = flop_voxel + sum(2 * 27 * 32 * 32 for _ in range(4)) + flop_voxel
+    for _ in range(2):
+        out, acts = net.forward(xt, 1, tshape, keep=True)
+        _, dy, _ = net.loss_grad(out, target, 1, tshape)
+        net.backward(acts, dy, 1, tshape)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out, acts = net.forward(xt, 1, tshape, keep=True)
+        _, dy, _ = net.loss_grad(out, target, 1, tshape)
+        gw, gb = net.backward(acts, dy, 1, tshape)
+        net.adam_step(gw, gb, 1e-6)
+    torch.cuda.synchronize(dev)
+    t_step = (time.perf_counter() - t0) / steps
+    tv = tshape[0] * tshape[1] * tshape[2]
+    step_tflops = flop_step_voxel * tv / t_step / 1e12
+    log(f"cnn training step at {tshape}: {t_step * 1e3:.2f} ms, {step_tflops:.0f} TFLOP/s", verbose)
+    line = {
+        "metric": "warm-start CNN inference time over the 36 (bin, echo) images of an 80x192x192 TVME problem",
+        "value": t_inf, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_inf * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic: random bf16 inputs, committed weights",
+        "config": {"workload": "5-layer 3x3x3 CNN, 32 channels, bf16 activations, fp32 accumulation on "
+                               "v_mfma_f32_32x32x8_bf16; inference 36 x 80x192x192 in chunks of 6; training "
+                               "steps (forward, MSE, backward, Adam) on one 64x96x96 image",
+                   "executed_flop_per_voxel_forward": flop_voxel,
+                   "training_step_ms": t_step * 1e3, "training_step_tflops": step_tflops},
+        "roofline": {"bound": "mfma", "achieved": inf_tflops, "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": inf_tflops / BF16_MFMA_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main(argv=None):
     args = parse_args(argv)
+    if args.config == "cnn":
+        return bench_cnn(args)
     if args.cpu_baseline_worker and args.config == "tvm48":
         tvm48_oracle_iteration()
         return 0
