@@ -154,9 +154,40 @@ struct FwdXArgs {
   float2* grid;
 };
 
+// Division of unsigned values below 2^31 by a divisor fixed at launch: one
+// high multiply, an add and a shift (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, fig. 4.1), which for
+// uniform operands stays on the scalar unit.
+struct FastDiv {
+  unsigned d, m, s;
+  __host__ __device__ unsigned div(unsigned x) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned hi = __umulhi(x, m);
+#else
+    const unsigned hi = (unsigned)(((unsigned long long)x * m) >> 32);
+#endif
+    return (unsigned)(((unsigned long long)hi + x) >> s);
+  }
+};
+
+inline FastDiv make_fastdiv(unsigned d) {
+  FastDiv f;
+  f.d = d;
+  f.s = 0;
+  while ((1ull << f.s) < d) ++f.s;
+  f.m = (unsigned)((((1ull << 32) * ((1ull << f.s) - d)) / d) + 1);
+  return f;
+}
+
 struct LinesArgs {
   FftAxis ax;
   unsigned xcd_chunk;  // workgroups per XCD run (whole lines of channel chunks), 0 = round-robin
+  // derived by launch_lines: block-index decoding without runtime divisions,
+  // and whether every position of a line lies within 2^31 bytes of the line's
+  // first position plus 15 position strides (32-bit buffer offsets)
+  FastDiv div_nchunk, div_inner, div_xcd;
+  unsigned xcd_full;  // blocks below this are remapped in XCD runs, the tail keeps its order
+  int off32;
   const float2* tw;
   float2* data;
   int K;
