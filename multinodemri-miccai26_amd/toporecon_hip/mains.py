@@ -17,6 +17,7 @@ import json
 import logging
 import os
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -251,16 +252,31 @@ def tvmw_main(argv=None) -> int:
 
 
 def cli_main(argv=None) -> int:
-    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS`` as in src/toporecon/cli.py:98-164."""
+    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS`` and ``stitch INPUT_DIR`` as in
+    src/toporecon/cli.py:98-177."""
     p = argparse.ArgumentParser(prog="toporecon_hip")
     p.add_argument("--verbose", action="store_true")
     sub = p.add_subparsers(dest="command", required=True)
     r = sub.add_parser("reconstruct")
     r.add_argument("--algorithm", choices=ALGORITHMS, required=True)
     r.add_argument("algorithm_arguments", nargs=argparse.REMAINDER)
+    s = sub.add_parser("stitch", help="Assemble reconstruction shards into one CFL/HDR image.")
+    s.add_argument("input_dir", type=Path)
+    s.add_argument("--input-prefix", "--stem", dest="stem",
+                   help="Shard prefix; default: output_stem from run_manifest.json.")
+    dest = s.add_mutually_exclusive_group()
+    dest.add_argument("--output-name", default="imout", help="Final filename inside INPUT_DIR; default: imout.")
+    dest.add_argument("--output", type=Path, help="Full path for the final image.")
+    s.add_argument("--overwrite", action="store_true")
     args = p.parse_args(argv)
     logging.basicConfig(level=logging.DEBUG if args.verbose else logging.INFO,
                         format="%(levelname)s: %(message)s")
+    if args.command == "stitch":
+        from .postprocessing import stitch
+        out = stitch(args.input_dir, stem=args.stem, output_name=args.output_name, output=args.output,
+                     overwrite=args.overwrite)
+        print(f"Wrote {out}.hdr and {out}.cfl")
+        return 0
     fwd = list(args.algorithm_arguments)
     if fwd[:1] == ["--"]:
         fwd = fwd[1:]
