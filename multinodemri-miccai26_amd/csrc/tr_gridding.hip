@@ -137,7 +137,9 @@ __global__ void __launch_bounds__(256) k_interp_v4(InterpArgs a, int G) {
   const int K = a.K;
   const int k2 = blockIdx.y * G + 2 * lane;         // first channel of this lane's pair
   const bool live = 2 * lane < G && k2 < K;
-  const int kk = min(k2, K - 2) >> 1;               // float4 index of the (clamped) pair
+  // float4 index of the lane's pair; a dead lane reads the group's first pair, a line the live lanes
+  // fetch anyway, so the gather touches no cache line outside the group's channels
+  const int kk = min(live ? k2 : (int)blockIdx.y * G, K - 2) >> 1;
   const TapRec& r = a.tap[m];
   const int packed = r.n;
   const int nx = (packed >> 8) & 15, ny = (packed >> 4) & 15;
@@ -235,8 +237,8 @@ __global__ void __launch_bounds__(256) k_interp_grp(InterpArgs a, int G) {
   const int K = a.K;
   const int k2 = blockIdx.y * G + 2 * lane;
   const bool live = 2 * lane < G && k2 < K;
-  const int kk = min(k2, K - 2) >> 1;
-  const TapRec* rec = a.tap + m0;
+  const int kk = min(live ? k2 : (int)blockIdx.y * G, K - 2) >> 1;  // dead lanes: the group's first pair
+  const TapRec* rec = a.tapg + m0;
   const int packed = rec[0].n;
   const int nx = (packed >> 8) & 15, ny = (packed >> 4) & 15;
   const int nz = (nx == 0 || ny == 0) ? 0 : (packed & 15);
@@ -600,7 +602,7 @@ int launch_interp(const InterpArgs& a, hipStream_t s) {
     const int ng = (K + 127) / 128;
     int G = (K + ng - 1) / ng;
     G += G & 1;
-    if (a.kgroups == 3 && a.grp && a.ngroups > 0)  // one wave per group of samples with equal taps
+    if (a.kgroups == 3 && a.grp && a.tapg && a.ngroups > 0)  // one wave per tap group
       hipLaunchKernelGGL(k_interp_grp, dim3(grid1d(a.ngroups, 4), ng), dim3(256), 0, s, a, G);
     else
       hipLaunchKernelGGL(k_interp_v4, dim3(grid1d(a.M, 4), ng), dim3(256), 0, s, a, G);

@@ -70,9 +70,13 @@ struct BinData {
   int* heavy_row = nullptr;      // heavy cluster ids
   int* heavy_seg_ptr = nullptr;  // nheavy + 1
   int64_t touched = 0;           // distinct grid points under the bin's taps
-  // groups of at most kTapGroup consecutive sorted samples with identical taps
+  // groups of at most kTapGroup consecutive sorted samples whose taps fit in
+  // one box of at most 5 positions per axis, gathered once per group; tapg
+  // holds every member's taps in its group's box (the box origin and extent,
+  // the member's weights shifted into it, zero where it has no tap)
   int64_t ngroups = 0;
   int* grp = nullptr;            // ngroups + 1 offsets into the sorted samples
+  TapRec* tapg = nullptr;
 };
 
 }  // namespace tr
@@ -232,6 +236,7 @@ constexpr int kTapGroup = 4;  // samples sharing one tap gather in the grouped i
 struct InterpArgs {
   GridGeom gg;
   const int* grp;   // tap groups (grouped mode)
+  const TapRec* tapg;  // per-sample taps in the group's box (grouped mode)
   int64_t ngroups;
   int64_t touched;  // distinct grid points read (algorithmic byte count only)
   int xcd_chunk;

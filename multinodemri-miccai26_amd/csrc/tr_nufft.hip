@@ -203,6 +203,7 @@ int tr_nufft_forward(tr_plan* p, int bin, const tr_c64* img, const tr_c64* maps,
   a.tap = b.tap;
   a.touched = b.touched;
   a.grp = b.grp;
+  a.tapg = b.tapg;
   a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;
@@ -272,6 +273,7 @@ int tr_data_dual_update(tr_plan* p, int bin, const tr_c64* ubar, const tr_c64* m
   a.tap = b.tap;
   a.touched = b.touched;
   a.grp = b.grp;
+  a.tapg = b.tapg;
   a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;
@@ -299,6 +301,7 @@ int tr_data_simulate(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   a.tap = b.tap;
   a.touched = b.touched;
   a.grp = b.grp;
+  a.tapg = b.tapg;
   a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;
@@ -323,6 +326,7 @@ int tr_data_residual(tr_plan* p, int bin, const tr_c64* x, const tr_c64* maps, i
   a.tap = b.tap;
   a.touched = b.touched;
   a.grp = b.grp;
+  a.tapg = b.tapg;
   a.ngroups = b.ngroups;
   a.M = b.M;
   a.K = E * C;

@@ -4,6 +4,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 
 #include "tr_internal.hpp"
@@ -93,18 +94,88 @@ __global__ void k_tap_records(const float4* kp, int64_t M, int3 g, float beta, T
   cnt[j] = count;
 }
 
-// flag[j] = 1 where sorted sample j's taps differ from sample j - 1's (first tap
-// and tap counts): the starts of runs of samples with identical taps.
-__global__ void k_tap_run_flags(const TapRec* tap, int64_t M, unsigned char* flag) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
-  bool start = j == 0;
-  if (!start) {
-    const TapRec& a = tap[j];
-    const TapRec& b = tap[j - 1];
-    start = a.n != b.n || a.p0[0] != b.p0[0] || a.p0[1] != b.p0[1] || a.p0[2] != b.p0[2];
+// Extends a group's box [P, P + E) (mod g) along one axis to hold taps
+// [p, p + n); false if the box would exceed 5 positions.
+static bool box_extend(int P, int E, int p, int n, int g, int& nP, int& nE) {
+  int o = p - P;
+  if (o < 0) o += g;
+  if (o + n <= 5) {
+    nP = P;
+    nE = std::max(E, o + n);
+    return true;
   }
-  flag[j] = start ? 1 : 0;
+  int o2 = P - p;
+  if (o2 < 0) o2 += g;
+  if (o2 + E <= 5) {
+    nP = p;
+    nE = std::max(E + o2, n);
+    return true;
+  }
+  return false;
+}
+
+// Tap groups of the grouped interpolation: runs of at most kTapGroup
+// consecutive sorted samples whose taps fit in one box of at most 5 positions
+// per axis (samples with no taps stay alone), greedily in sorted order; and
+// b.tapg, each member's record rewritten to its group's box.  Shifting a
+// member's weights into the box keeps the order of its nonzero terms, and the
+// added terms have weight 0, so the gathered sums equal the per-sample ones.
+static int build_tap_groups(BinData& b, int64_t M, int3 g3, hipStream_t s) {
+  std::vector<TapRec> h((size_t)M);
+  TR_HIP(hipMemcpyAsync(h.data(), b.tap, sizeof(TapRec) * M, hipMemcpyDeviceToHost, s));
+  TR_HIP(hipStreamSynchronize(s));
+  const int gdim[3] = {g3.x, g3.y, g3.z};
+  auto ntaps = [](const TapRec& r, int ax) { return (r.n >> (4 * ax)) & 15; };
+  std::vector<int> starts;
+  starts.reserve((size_t)M / 2 + 2);
+  std::vector<TapRec> u(h);
+  int64_t j = 0;
+  while (j < M) {
+    int P[3], E[3];
+    bool empty = false;
+    for (int ax = 0; ax < 3; ++ax) {
+      P[ax] = h[j].p0[ax];
+      E[ax] = ntaps(h[j], ax);
+      empty |= E[ax] == 0;
+    }
+    int64_t k = j + 1;
+    for (; !empty && k < M && k - j < kTapGroup; ++k) {
+      int nP[3], nE[3];
+      bool fits = true;
+      for (int ax = 0; ax < 3 && fits; ++ax) {
+        const int n = ntaps(h[k], ax);
+        fits = n > 0 && box_extend(P[ax], E[ax], h[k].p0[ax], n, gdim[ax], nP[ax], nE[ax]);
+      }
+      if (!fits) break;
+      for (int ax = 0; ax < 3; ++ax) {
+        P[ax] = nP[ax];
+        E[ax] = nE[ax];
+      }
+    }
+    starts.push_back((int)j);
+    if (!empty && k - j > 1) {
+      for (int64_t i = j; i < k; ++i) {
+        TapRec& r = u[(size_t)i];
+        for (int ax = 0; ax < 3; ++ax) {
+          const int n = ntaps(h[i], ax);
+          int sh = h[i].p0[ax] - P[ax];
+          if (sh < 0) sh += gdim[ax];
+          for (int d = 0; d < 5; ++d)
+            r.w[5 * ax + d] = (d >= sh && d < sh + n) ? h[i].w[5 * ax + d - sh] : 0.0f;
+          r.p0[ax] = P[ax];
+        }
+        r.n = E[0] | (E[1] << 4) | (E[2] << 8);
+      }
+    }
+    j = k;
+  }
+  starts.push_back((int)M);
+  b.ngroups = (int64_t)starts.size() - 1;
+  TR_HIP(hipMalloc(&b.grp, sizeof(int) * starts.size()));
+  TR_HIP(hipMemcpy(b.grp, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice));
+  TR_HIP(hipMalloc(&b.tapg, sizeof(TapRec) * M));
+  TR_HIP(hipMemcpy(b.tapg, u.data(), sizeof(TapRec) * M, hipMemcpyHostToDevice));
+  return kOk;
 }
 
 // Distinct clusters touched along one axis, with the weight of every cluster
@@ -320,29 +391,7 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   TR_HIP(hipMemsetAsync(cnt + M, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_tap_records, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, g3, p->beta, b.tap, cnt);
   TR_LAUNCH_CHECK();
-  {  // groups of up to kTapGroup consecutive samples with identical taps
-    unsigned char* flag = nullptr;
-    TR_HIP(hipMalloc(&flag, (size_t)M));
-    hipLaunchKernelGGL(k_tap_run_flags, dim3(grid1d(M, 256)), dim3(256), 0, s, b.tap, M, flag);
-    std::vector<unsigned char> hf((size_t)M);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(hf.data(), flag, (size_t)M, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(flag);
-    TR_HIP(e);
-    std::vector<int> starts;
-    starts.reserve((size_t)M / 2 + 2);
-    int run = 0;
-    for (int64_t j = 0; j < M; ++j) {
-      if (hf[(size_t)j] || run == kTapGroup) run = 0;
-      if (run == 0) starts.push_back((int)j);
-      ++run;
-    }
-    starts.push_back((int)M);
-    b.ngroups = (int64_t)starts.size() - 1;
-    TR_HIP(hipMalloc(&b.grp, sizeof(int) * starts.size()));
-    TR_HIP(hipMemcpy(b.grp, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice));
-  }
+  if (int r = build_tap_groups(b, M, g3, s)) return r;
   size_t scan_bytes = 0;
   TR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, (int)(M + 1), s));
   void* scan_tmp = nullptr;
@@ -506,6 +555,7 @@ void free_plan(tr_plan* p) {
     (void)hipFree(b.kp);
     (void)hipFree(b.tap);
     (void)hipFree(b.grp);
+    (void)hipFree(b.tapg);
     (void)hipFree(b.row_ptr);
     (void)hipFree(b.pairs);
     (void)hipFree(b.seg_row);
