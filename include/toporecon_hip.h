@@ -90,7 +90,8 @@ int tr_plan_release_workspace(tr_plan* plan);
  * (group-major), shrinking the grid footprint each XCD's L2 must hold; 2 does the same with
  * 16-byte loads (two channels per lane, groups of up to 128 channels; needs an even channel
  * count >= 96 and 16-byte aligned buffers, else it runs as 1); 3 is 2 with one tap gather per
- * group of up to 4 consecutive sorted samples that share their taps (dense k-space). */
+ * group of up to 4 consecutive sorted samples whose taps fit one box of 5 x 5 x 5 grid points
+ * (dense k-space). */
 #define TR_PARAM_INTERP_KGROUPS 6
 /* TR_PARAM_WORKSPACE_SLOTS (1..8, default 1): calls for bin b use grid workspace b % slots, so
  * calls for bins in different slots may run concurrently on different streams (each slot holds

@@ -212,11 +212,14 @@ __global__ void __launch_bounds__(256) k_interp_v4(InterpArgs a, int G) {
 
 // ---------------------------------------------------------------------------
 // Grouped gather: one wave per group of up to kTapGroup consecutive sorted
-// samples with identical taps (same first tap and tap counts; dense k-space
-// regions hold many), 16-byte loads as k_interp_v4.  Each tap is loaded once
-// for the group and added into every member's sums with the member's own
-// weights, in the same (dz, dy, dx) order and with the same products as the
-// per-sample kernels, so every sample's result is bit-identical to theirs.
+// samples whose taps fit in one box of at most 5 positions per axis (dense
+// k-space regions hold many), 16-byte loads as k_interp_v4.  The group's
+// records (a.tapg) give every member the box's origin and extent and the
+// member's weights shifted into the box, zero where it has no tap.  Each tap
+// is loaded once for the group and added into every member's sums with the
+// member's own weights, in the same (dz, dy, dx) order and with the same
+// products as the per-sample kernels; the extra terms have weight 0, so every
+// sample's result is bit-identical to theirs.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void acc4(float4& acc, float w, const float4& u) {
   acc.x += w * u.x;
