@@ -252,8 +252,8 @@ def tvmw_main(argv=None) -> int:
 
 
 def cli_main(argv=None) -> int:
-    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS`` and ``stitch INPUT_DIR`` as in
-    src/toporecon/cli.py:98-177."""
+    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS``, ``stitch INPUT_DIR`` and
+    ``resp INPUT --work-dir DIR`` as in src/toporecon/cli.py:75-183."""
     p = argparse.ArgumentParser(prog="toporecon_hip")
     p.add_argument("--verbose", action="store_true")
     sub = p.add_subparsers(dest="command", required=True)
@@ -268,6 +268,9 @@ def cli_main(argv=None) -> int:
     dest.add_argument("--output-name", default="imout", help="Final filename inside INPUT_DIR; default: imout.")
     dest.add_argument("--output", type=Path, help="Full path for the final image.")
     s.add_argument("--overwrite", action="store_true")
+    rp = sub.add_parser("resp", help="Estimate resp.hdr/.cfl in a separate work directory.")
+    rp.add_argument("input", type=Path)
+    rp.add_argument("--work-dir", type=Path, required=True)
     args = p.parse_args(argv)
     logging.basicConfig(level=logging.DEBUG if args.verbose else logging.INFO,
                         format="%(levelname)s: %(message)s")
@@ -275,6 +278,12 @@ def cli_main(argv=None) -> int:
         from .postprocessing import stitch
         out = stitch(args.input_dir, stem=args.stem, output_name=args.output_name, output=args.output,
                      overwrite=args.overwrite)
+        print(f"Wrote {out}.hdr and {out}.cfl")
+        return 0
+    if args.command == "resp":
+        from .preprocessing import estimate_respiratory_from_dataset
+        args.work_dir.mkdir(parents=True, exist_ok=True)
+        out = estimate_respiratory_from_dataset(args.input, args.work_dir)
         print(f"Wrote {out}.hdr and {out}.cfl")
         return 0
     fwd = list(args.algorithm_arguments)
