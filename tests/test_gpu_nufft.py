@@ -91,13 +91,17 @@ def test_batched_coils_echoes(torch_mod, E, C):
     assert _rel(adj, ref) < TOL
 
 
-@pytest.mark.parametrize("shape", [(8, 16, 192), (8, 192, 8), (80, 8, 8), (32, 48, 64), (16, 96, 48)])
-def test_two_level_fft_lengths(torch_mod, shape):
-    """16-coil / 32-channel batches on grids whose lengths (240, 100, 80, 60, 40, 120) take the
-    two-level register FFT in the x, y and z passes, forward and density-weighted adjoint."""
+@pytest.mark.parametrize("shape,E,C", [((8, 16, 192), 2, 16), ((8, 192, 8), 2, 16), ((80, 8, 8), 2, 16),
+                                       ((32, 48, 64), 2, 16), ((16, 96, 48), 2, 16),
+                                       ((8, 16, 192), 3, 8), ((80, 8, 8), 3, 8), ((16, 96, 48), 3, 8)])
+def test_two_level_fft_lengths(torch_mod, shape, E, C):
+    """Batches on grids whose lengths (240, 100, 80, 60, 40, 120) take the two-level register FFT
+    in the x, y and z passes, forward and density-weighted adjoint: 32 channels (whole 16-channel
+    chunks) and 24 channels (a half-empty second chunk, whose dead lanes the y / z passes mask
+    with out-of-range buffer offsets)."""
     from toporecon_hip.nufft_backend import HipNufftBackend
-    print("two-level", shape, flush=True)
-    E, C, M = 2, 16, 3000
+    print("two-level", shape, E, C, flush=True)
+    M = 3000
     rng = np.random.default_rng(sum(shape))
     coords = [_coords(rng, shape, M)]
     be = HipNufftBackend(shape, coords, 0)
