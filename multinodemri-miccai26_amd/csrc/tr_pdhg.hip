@@ -50,6 +50,30 @@ __device__ __forceinline__ void motion_dual(const VolGeom& g, int b, int64_t v, 
                                             float2* p_m, float sigma, float lam, int l2) {
   const int64_t EN = (int64_t)g.E * g.N;
   const int64_t base = (int64_t)b * EN + v;
+  constexpr int kRegE = 8;
+  if (l2 && g.E <= kRegE) {
+    // the echoes' ascended values stay in registers until the coupled norm is known, so p_m is
+    // read and written once
+    float2 pe[kRegE];
+    float ss = 0.f;
+#pragma unroll
+    for (int e = 0; e < kRegE; ++e) {
+      if (e < g.E) {
+        const int64_t i = base + (int64_t)e * g.N;
+        const float2 u0 = ubar[i];
+        float2 d = make_float2(0.f, 0.f);
+        if (b < g.B - 1) d = csub(ubar[i + EN], u0);
+        const float2 p = p_m[i];
+        pe[e] = make_float2(p.x + sigma * d.x, p.y + sigma * d.y);
+        ss += cabs2(pe[e]);
+      }
+    }
+    const float den = fmaxf(1.0f, sqrtf(ss) / lam);
+#pragma unroll
+    for (int e = 0; e < kRegE; ++e)
+      if (e < g.E) p_m[base + (int64_t)e * g.N] = make_float2(pe[e].x / den, pe[e].y / den);
+    return;
+  }
   float ss = 0.f;
   for (int e = 0; e < g.E; ++e) {
     const int64_t i = base + (int64_t)e * g.N;
