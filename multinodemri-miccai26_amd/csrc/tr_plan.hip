@@ -33,6 +33,7 @@ __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh,
   // with the same taps (same first tap and tap counts, as axis_taps computes them) are adjacent
   // in the sorted order and the interpolation can gather their taps once.
   unsigned tile = 0, local = 0, five = 0;
+  unsigned tq[3] = {0u, 0u, 0u};
   const float kv[3] = {kz, ky, kx};
   const int ntv[3] = {nt.x, nt.y, nt.z};
   for (int ax = 0; ax < 3; ++ax) {
@@ -45,10 +46,14 @@ __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh,
     int t = (int)floor((t0 + 4) / (double)kTile);
     t = t < 0 ? 0 : (t >= ntv[ax] ? ntv[ax] - 1 : t);
     const int off = t0 + 4 - t * kTile;  // in [0, kTile) unless clamped
-    tile = tile * (unsigned)ntv[ax] + (unsigned)t;
+    tq[ax] = (unsigned)t;
     local = local * (unsigned)(kTile + 2) + (unsigned)(off < 0 ? 0 : (off > kTile ? kTile + 1 : off));
     five = five * 2u + (n == 5 ? 1u : 0u);
   }
+  // tiles in Z order (bits of z, y, x interleaved), so runs of consecutive tiles cover compact
+  // blocks of the grid rather than rows
+  for (int bit = 9; bit >= 0; --bit)
+    tile = (tile << 3) | (((tq[0] >> bit) & 1u) << 2) | (((tq[1] >> bit) & 1u) << 1) | ((tq[2] >> bit) & 1u);
   key[m] = (tile * (unsigned)((kTile + 2) * (kTile + 2) * (kTile + 2)) + local) * 8u + five;
   idx[m] = (int)m;
 }
@@ -371,7 +376,9 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
                      kp_raw, key_in, idx);
   TR_LAUNCH_CHECK();
   int key_bits = 1;
-  while (((int64_t)1 << key_bits) < (int64_t)nt.x * nt.y * nt.z * (kTile + 2) * (kTile + 2) * (kTile + 2) * 8)
+  const int64_t zspan = (int64_t)pow2ceil(std::max(nt.x, std::max(nt.y, nt.z)));  // Z-order tile range per axis
+  if (zspan > 1024) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
+  while (((int64_t)1 << key_bits) < zspan * zspan * zspan * (kTile + 2) * (kTile + 2) * (kTile + 2) * 8)
     ++key_bits;
   if (key_bits > 32) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
   size_t tmp_bytes = 0;
