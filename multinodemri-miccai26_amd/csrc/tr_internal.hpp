@@ -174,7 +174,7 @@ struct FastDiv {
   }
 };
 
-inline FastDiv make_fastdiv(unsigned d) {
+inline FastDiv make_fastdiv(unsigned d) {  // d >= 1
   FastDiv f;
   f.d = d;
   f.s = 0;

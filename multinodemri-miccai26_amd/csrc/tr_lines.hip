@@ -191,6 +191,7 @@ struct LinesLaunch {
 };
 
 int launch_lines(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const LinesArgs& a0) {
+  if (a0.K <= 0 || grid.x == 0) return kOk;  // nothing to transform (and no divisor of 0 below)
   LinesArgs a = a0;
   const unsigned S = a.xcd_chunk;
   a.div_nchunk = make_fastdiv((unsigned)((a.K + kc - 1) / kc));
