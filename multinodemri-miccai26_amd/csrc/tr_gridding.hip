@@ -426,20 +426,47 @@ __device__ __forceinline__ void cl_store(const SpreadArgs& a, int64_t cl, int k,
                                          const float2 (&acc)[ClLanes<KCI>::NP]) {
   constexpr int NPL = ClLanes<KCI>::NPL;
   if (k >= a.K) return;
-  const int ncy = a.ncl[1], ncx = a.ncl[2];
-  const int cz = (int)(cl / ((int64_t)ncy * ncx));
-  const int rem = (int)(cl - (int64_t)cz * ncy * ncx);
-  const int cy = rem / ncx, cx = rem - (rem / ncx) * ncx;
   const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
+  if constexpr (NPL == 1) {
+    // one wave per (cluster, 64 channels): everything but the channel is wave-uniform, so each
+    // (y, x) column reads its z window once and its kClZ points sit at one uniform stride from a
+    // column base (scalar registers); per point that leaves a window test and a store.  Cluster
+    // coordinates come from precomputed inverses of the cluster-grid sizes.
+    const unsigned ucl = (unsigned)cl;
+    const int cz = (int)a.div_plane.div(ucl);
+    const unsigned rem = ucl - (unsigned)cz * a.div_plane.d;
+    const int cy = (int)a.div_ncx.div(rem), cx = (int)(rem - (unsigned)cy * a.div_ncx.d);
+    const int64_t zs = (int64_t)gy * gx * a.K;
 #pragma unroll
-  for (int i = 0; i < ClLanes<KCI>::NP; ++i) {
-    const int pt = pl + i * NPL;
-    if (pt >= kClPts) continue;
-    const int z = cz * kClZ + pt / (kClY * kClX);
-    const int y = cy * kClY + (pt / kClX) % kClY;
-    const int x = cx * kClX + pt % kClX;
-    if (z < gz && y < gy && x < gx && (!a.col_win || in_window(z, a.col_win[y * gx + x], gz)))
-      a.grid[(((int64_t)z * gy + y) * gx + x) * a.K + k] = acc[i];
+    for (int iy = 0; iy < kClY; ++iy) {
+#pragma unroll
+      for (int ix = 0; ix < kClX; ++ix) {
+        const int y = cy * kClY + iy, x = cx * kClX + ix;
+        if (y >= gy || x >= gx) continue;
+        const int2 w = a.col_win ? a.col_win[y * gx + x] : make_int2(0, gz);
+        float2* col = a.grid + ((int64_t)y * gx + x) * a.K + (int64_t)(cz * kClZ) * zs;
+#pragma unroll
+        for (int iz = 0; iz < kClZ; ++iz) {
+          const int z = cz * kClZ + iz;
+          if (z < gz && in_window(z, w, gz)) col[iz * zs + k] = acc[(iz * kClY + iy) * kClX + ix];
+        }
+      }
+    }
+  } else {
+    const int ncy = a.ncl[1], ncx = a.ncl[2];
+    const int cz = (int)(cl / ((int64_t)ncy * ncx));
+    const int rem = (int)(cl - (int64_t)cz * ncy * ncx);
+    const int cy = rem / ncx, cx = rem - (rem / ncx) * ncx;
+#pragma unroll
+    for (int i = 0; i < ClLanes<KCI>::NP; ++i) {
+      const int pt = pl + i * NPL;
+      if (pt >= kClPts) continue;
+      const int z = cz * kClZ + pt / (kClY * kClX);
+      const int y = cy * kClY + (pt / kClX) % kClY;
+      const int x = cx * kClX + pt % kClX;
+      if (z < gz && y < gy && x < gx && (!a.col_win || in_window(z, a.col_win[y * gx + x], gz)))
+        a.grid[(((int64_t)z * gy + y) * gx + x) * a.K + k] = acc[i];
+    }
   }
 }
 
@@ -448,11 +475,16 @@ template <int KCI, int SORTED>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs a) {
   constexpr int NP = ClLanes<KCI>::NP;
   const int lane = threadIdx.x & 63;
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
-  const int64_t item = (int64_t)blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (item >= a.NCL * a.ngrp) return;
-  const int64_t cl = item / a.ngrp;
-  const int grp = (int)(item - cl * a.ngrp);
+  unsigned blk = blockIdx.x;
+  if (blk < a.xcd_full) {  // runs of xcd_chunk consecutive logical blocks per XCD
+    const unsigned r = blk / kNumXcd, x = blk % kNumXcd, q = a.div_xcd.div(r);
+    blk = q * (kNumXcd * (unsigned)a.xcd_chunk) + x * (unsigned)a.xcd_chunk + (r - q * (unsigned)a.xcd_chunk);
+  }
+  const unsigned item = blk * 4u + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= (unsigned)(a.NCL * a.ngrp)) return;
+  const unsigned ucl = a.div_ngrp.div(item);
+  const int64_t cl = ucl;
+  const int grp = (int)(item - ucl * (unsigned)a.ngrp);
   const int j0 = a.row_ptr[cl], j1 = a.row_ptr[cl + 1];
   if (j1 - j0 > kSpreadSeg) return;  // heavy cluster: written by k_spread_combine
   const int k = grp * KCI + lane % KCI;
@@ -653,6 +685,10 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
   a.row_ptr = b.row_ptr;
   a.pairs = b.pairs;
   a.ngrp = (K + kci - 1) / kci;
+  if (a.NCL * a.ngrp >= ((int64_t)1 << 31)) TR_FAIL(kUnsupported, "too many grid clusters for the spreading");
+  a.div_ngrp = make_fastdiv((unsigned)a.ngrp);
+  a.div_plane = make_fastdiv((unsigned)(p->ncl[1] * p->ncl[2]));
+  a.div_ncx = make_fastdiv((unsigned)p->ncl[2]);
   a.nseg = b.nseg;
   a.seg_row = b.seg_row;
   a.seg_j0 = b.seg_j0;
@@ -682,6 +718,11 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
     if (a.sorted_src) TR_SPREAD_KCI(KERN, 1, n);        \
     else TR_SPREAD_KCI(KERN, 0, n);                     \
   } while (0)
+  {
+    const unsigned S = (unsigned)a.xcd_chunk, nb = grid1d(a.NCL * a.ngrp, 4);
+    a.div_xcd = make_fastdiv(S ? S : 1u);
+    a.xcd_full = S ? nb / (kNumXcd * S) * (kNumXcd * S) : 0u;
+  }
   TR_SPREAD_ALL(k_spread, a.NCL);
   if (b.nseg > 0) {
     TR_SPREAD_ALL(k_spread_segments, b.nseg);

@@ -283,6 +283,10 @@ struct SpreadArgs {
   const int* heavy_seg_ptr;
   float2* partial;  // nseg * kClPts * K
   int ngrp;         // channel groups per cluster
+  // derived by launch_spread for the light-cluster kernel: block and cluster decoding with
+  // precomputed multiplicative inverses (uniform values stay on the scalar unit)
+  FastDiv div_ngrp, div_plane, div_ncx, div_xcd;
+  unsigned xcd_full;  // blocks below this are remapped in XCD runs of xcd_chunk
   const int2* col_win;  // support windows (null = write every point)
 };
 
