@@ -5,25 +5,36 @@ Workload (default): TVME PDHG to tol 1e-3 (max 300 iterations, sigma = tau =
 1/6, lambda_motion = lambda_echo = 1e-5, L2 echo coupling) at 80x192x192,
 32 coils, 6 echoes (TE = 1.0 + 1.3 e ms), 6 motion bins, 9000 cone
 interleaves x 512 samples per echo (readout fraction 0.98, 2/98 % respiratory
-outlier drop), synthetic seeded data simulated on the GPU.  A step is one
-full solve; inputs are resident in HBM when the timed region starts.
+outlier drop), synthetic seeded data simulated on the GPU.  Inputs are
+resident in HBM when a timed region starts.
 
-Two legs run on the same problem: plain PDHG from zeros (as the reference),
-and the learned warm start (density-compensated adjoint -> 3-D CNN on MFMA ->
-per-(bin, echo) data-fitted scale -> 4 conjugate-gradient data-consistency
-steps -> PDHG from there to the same tolerance, the warm-start work inside the
-timed region).  ``value`` is the learned leg's time-to-tol when its final
-objective is within +-0.1 % of the plain leg's, otherwise the plain leg's (the
-fallback); both legs are reported.  Variant starts (CNN without CG, CG from
-zeros without the CNN, CNN with 3 or 5 CG steps) are timed too and reported
-with their signed objective differences; they never set ``value``.
+A step is one time-to-tolerance solve of the headline leg: the learned warm
+start (density-compensated adjoint -> 3-D CNN on MFMA -> per-(bin, echo)
+data-fitted scale -> 4 conjugate-gradient data-consistency steps) followed by
+PDHG to the tolerance, the warm-start work inside the timed region.  The
+plain leg (PDHG from zeros, as the reference runs it) is timed once as the
+denominator.  ``value`` is the learned leg's mean time-to-tol over the timed
+steps when its final objective is within +-0.1 % of the plain leg's (plain
+PDHG is the fallback otherwise, and then ``value`` is the plain time).
+Warmup steps are learned solves capped at a few PDHG iterations.  Variant
+starts (``--variants``) are off by default.  A wall-clock guard stops the
+timed steps early (and records how many ran) if the run would pass
+``--budget-s`` seconds since the process started.
 
-Prints one JSON line: time-to-tol (value, seconds), iterations, per-kernel
-algorithmic GB/s from HIP events recorded on the launch stream (plain leg),
-the dominant kernel's fraction of the 8 TB/s HBM peak with its measured HBM
-traffic, forward/adjoint NUFFT GB/s, and a single-core CPU baseline (the
-float64 oracle timed on a bounded sample and extrapolated to the plain
-iteration count).
+Per-kernel times come from HIP events that the library records on each
+kernel's own launch stream during the timed steps (bins on one stream, so no
+two kernels overlap); ``roofline`` reports the dominant kernel against the
+8 TB/s HBM peak on the algorithmic-byte model (per-unit bytes x units per
+launch) and, labelled separately, on the bytes the kernel must move after
+k-space support pruning.  ``traffic`` is replayed from the counter profile
+named in ``traffic_source`` (two rocprofv3 --pmc runs of this bench).
+
+The CPU baseline is the single-precision NumPy restatement of the reference's
+``--device -1`` path (oracle/sigpy32.py: SigPy's nufft / nufft_adjoint with
+numpy.fft, one thread), timed on one (coil, echo) channel of every motion bin
+plus a z-slab of the TV element-wise work, extrapolated to a full iteration
+and to the plain solve's iteration count; and the same channels spread over
+all host cores.
 """
 
 from __future__ import annotations
@@ -34,7 +45,10 @@ import math
 import os
 import subprocess
 import sys
+import tempfile
 import time
+
+T_PROCESS0 = time.perf_counter()
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(REPO, "multinodemri-miccai26_amd")
@@ -45,38 +59,46 @@ for _p in (REPO, PKG):
 METRIC = "TVME time-to-tol 1e-3 (learned warm start vs plain) on 1 GPU; NUFFT GB/s"
 HBM_PEAK_GBPS = 8000.0
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic.json")
+WARMUP_ITERS = 3          # PDHG iterations of a warmup solve
+DEFAULT_BUDGET_S = 450.0  # wall-clock guard from process start (the driver kills at 600 s)
+CPU_BUDGET_S = 45.0       # single-core CPU-baseline sample
+EW_SLAB_Z = 8             # z planes of the element-wise CPU sample
 
 
 def parse_args(argv=None):
-    ap = argparse.ArgumentParser(description=__doc__)
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", choices=("full", "small", "batched", "tvm48", "cnn"), default="full",
                     help="full: TVME 80x192x192 (the headline); small: a reduced TVME; batched: 12 uneven "
                          "TVM instances, HIP-graph batch vs sequential solves; tvm48: plain TVM at "
-                         "48x64x64, 8 coils, 2 echoes, 4 bins beside one full float64 oracle iteration; cnn: "
-                         "the warm-start network's MFMA convolutions (inference at 80x192x192, training "
-                         "steps at 64x96x96)")
+                         "48x64x64, 8 coils, 2 echoes, 4 bins beside one full single-precision CPU "
+                         "iteration; cnn: the warm-start network's MFMA convolutions")
     ap.add_argument("--max-iter", type=int, default=None, help="override max_iter (profiling runs)")
     ap.add_argument("--tol", type=float, default=None)
+    ap.add_argument("--budget-s", type=float, default=DEFAULT_BUDGET_S,
+                    help="stop the timed steps early when the run would pass this many seconds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-baseline-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-baseline-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--no-timed-events", action="store_true",
+                    help="no per-kernel HIP events in the timed steps (no roofline)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarize two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, "
                          "WRITE_SIZE) of a run with --copy-calibration-mib into profiles/traffic.json")
+    ap.add_argument("--traffic-command", default=None,
+                    help="with --traffic-from: the bench command the counters were collected on")
     ap.add_argument("--params", default=None,
                     help="JSON launch parameters (interp_xcd_chunk, interp_kgroups, spread_xcd_chunk, "
-                         "lines_xcd_lines, bin_streams) to set instead of autotuning "
-                         "(profiling runs)")
+                         "lines_xcd_lines) to set instead of autotuning (profiling runs)")
     ap.add_argument("--legs", choices=("all", "plain"), default="all",
-                    help="plain: only the plain PDHG leg (no warm start, no power iteration)")
+                    help="plain: only the plain PDHG leg, timed --steps times (no warm start)")
     ap.add_argument("--cg-steps", type=int, default=None,
                     help="CG steps of the learned leg (default: the warm-start module's default, 4)")
-    ap.add_argument("--variants", default="cnn+0,zero+4,cnn+3,cnn+5",
-                    help="comma list of extra warm starts 'cnn+K' / 'zero+K' (K CG steps) timed and "
-                         "reported beside the learned leg; '' for none")
+    ap.add_argument("--variants", default="",
+                    help="comma list of extra warm starts 'cnn+K' / 'zero+K' (K CG steps) timed once each "
+                         "and reported beside the learned leg, e.g. cnn+0,zero+4,cnn+3,cnn+5")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the solve, copy a buffer of this many MiB once (a kernel of known "
                          "traffic for scaling hardware byte counters)")
@@ -85,7 +107,39 @@ def parse_args(argv=None):
 
 def log(msg, verbose=True):
     if verbose:
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+        print(f"[bench {time.perf_counter() - T_PROCESS0:6.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
+def work_plan(args, variants=None):
+    """What a run of the full configuration does, in solves (used by main and checked by a test).
+
+    warmup_solves: learned solves capped at WARMUP_ITERS iterations; plain_solves: 1 (all with
+    --legs plain: --steps); learned_solves: --steps (0 with --legs plain); variant_solves: one per
+    --variants entry; power_iterations: the ||K|| estimate; cpu: single-core sample bounded by
+    CPU_BUDGET_S plus one concurrent round on all host cores."""
+    variants = [v for v in (x.strip() for x in (args.variants or "").split(",")) if v] if variants is None \
+        else variants
+    plain_only = args.legs == "plain"
+    return {"warmup_solves": args.warmup, "warmup_iterations": WARMUP_ITERS,
+            "plain_solves": args.steps if plain_only else 1,
+            "learned_solves": 0 if plain_only else args.steps,
+            "variant_solves": 0 if plain_only else len(variants),
+            "power_iterations": 0 if plain_only else 20,
+            "cpu_single_core_budget_s": 0.0 if args.no_cpu_baseline else CPU_BUDGET_S,
+            "cpu_all_cores_rounds": 0 if args.no_cpu_baseline else 1,
+            "budget_s": args.budget_s}
+
+
+def estimate_seconds(plan, *, t_setup, t_iter, t_start, n_plain, n_learned, t_cpu_round):
+    """Wall-clock estimate of a run: t_setup (imports, data, plan, autotune), t_iter (one PDHG
+    iteration), t_start (the warm-start work), iteration counts of the two legs, t_cpu_round (one
+    concurrent all-cores CPU round); a power iteration costs about two PDHG iterations."""
+    learned = t_start + n_learned * t_iter
+    return (t_setup + plan["warmup_solves"] * (t_start + plan["warmup_iterations"] * t_iter)
+            + plan["plain_solves"] * n_plain * t_iter
+            + (plan["learned_solves"] + plan["variant_solves"]) * learned
+            + plan["power_iterations"] * 2 * t_iter
+            + plan["cpu_single_core_budget_s"] + plan["cpu_all_cores_rounds"] * t_cpu_round)
 
 
 def make_config(name):
@@ -100,8 +154,8 @@ def make_config(name):
 
 
 def nufft_model_bytes(plan_samples, E, C, img_shape, grid_shape):
-    """Algorithmic bytes of one forward and one adjoint pass over all bins, on the unpruned model
-    (every grid pass moves the whole oversampled grid), for the NUFFT GB/s figures:
+    """Algorithmic bytes of one forward and one adjoint pass over all bins (every grid pass moves the
+    whole oversampled grid), for the NUFFT GB/s figures:
 
     forward per bin: pad 8EN + 8CN + 8KG, fft3 16KG, interp 8KG + 12M + 24KM;
     adjoint per bin: spread 8KM + 16M + 8KG, ifft3 16KG, crop 8KN + 8CN + 8EN.
@@ -116,82 +170,179 @@ def nufft_model_bytes(plan_samples, E, C, img_shape, grid_shape):
     return fwd, adj
 
 
-def cpu_baseline_worker(cfg_name):
-    """Time the float64 oracle on one bin x one channel forward + adjoint and one bin's TV steps."""
+def model_bytes_per_launch(name, M, E, C, N, G, B):
+    """Algorithmic bytes of one launch of a logical kernel on the per-unit model (each input read once,
+    each output written once, every grid pass over the whole oversampled grid).  M: samples of the
+    launch's bin; the line passes are one 1-D FFT pass over the grid (read + write)."""
+    K = E * C
+    return {
+        "fwd_x_pad": 8 * E * N + 8 * C * N + 8 * K * G,
+        "fft_y_fwd": 16 * K * G, "fft_z_fwd": 16 * K * G, "fft_z_inv": 16 * K * G, "fft_y_inv": 16 * K * G,
+        "interp": 8 * K * G + 12 * M + 24 * K * M,
+        "spread": 8 * K * M + 16 * M + 8 * K * G,
+        "adj_x_crop": 8 * K * N + 8 * C * N + 8 * E * N,
+        "tvme_dual": 9 * 8 * B * E * N, "tvme_primal": 8 * 8 * B * E * N,
+    }.get(name)
+
+
+# ---------------------------------------------------------------------------------------------
+# CPU baseline (single-precision NumPy restatement of the reference's --device -1 path)
+# ---------------------------------------------------------------------------------------------
+
+def _cpu_worker(job_path):
+    """Time the jobs of one CPU-baseline worker; prints one JSON line."""
     import numpy as np
 
-    from oracle import nufft as on
-    from oracle import pdhg as op
-    from toporecon_hip import synth
-    cfg = make_config(cfg_name)
-    t0 = time.perf_counter()
-    acq = synth.make_acquisition(cfg)
-    idx = acq.bin_index(0)
-    coord = acq.coord[idx].reshape(-1, 3)
-    M = coord.shape[0]
-    img = (acq.truth[0, 0] * acq.mps[0]).astype(np.complex128)
-    gen_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    y = on.nufft(img, coord)
-    t_fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    on.nufft_adjoint(y * acq.dcf[idx].reshape(-1), coord, cfg.img_shape)
-    t_adj = time.perf_counter() - t0
-    # element-wise TVME work of one bin (all echoes), float64
-    u = acq.truth[:1].astype(np.complex128)
-    t0 = time.perf_counter()
-    de = op.echo_diff(u)
-    px, py, pz = op.prox_iso3(op.fdiff(de, -1), op.fdiff(de, -2), op.fdiff(de, -3), cfg.lambda_echo)
-    pm = op.prox_motion(op.motion_diff(u), cfg.lambda_motion, True)
-    divp = op.fdiff_adj(px, -1) + op.fdiff_adj(py, -2) + op.fdiff_adj(pz, -3)
-    _ = u - (1 / 6) * op.motion_div_tvme(pm) - (1 / 6) * op.echo_div(divp)
-    t_ew = time.perf_counter() - t0
-    K = cfg.echoes * cfg.coils
-    per_iter = cfg.bins * (K * (t_fwd + t_adj) + t_ew)
-    print(json.dumps({"t_fwd": t_fwd, "t_adj": t_adj, "t_ew_bin": t_ew, "per_iter": per_iter,
-                      "samples": M, "K": K, "bins": cfg.bins, "gen_s": gen_s}))
+    from oracle import sigpy32 as s32
+    with open(job_path) as f:
+        job = json.load(f)
+    if job.get("core") is not None:
+        os.sched_setaffinity(0, {int(job["core"])})
+    d = job["dir"]
+    shape = tuple(job["shape"])
+    rng = np.random.default_rng(job.get("seed", 0))
+    while time.time() < job.get("start_at", 0.0):
+        time.sleep(0.01)
+    t_begin = time.perf_counter()
+    out = {"channels": [], "ew_slab_s": None}
+    for ch in job["channels"]:
+        if time.perf_counter() - t_begin > job.get("budget_s", 1e9):
+            break
+        b, c, e = ch["bin"], ch["coil"], ch["echo"]
+        coord = np.load(os.path.join(d, f"coord_{b}.npy"), mmap_mode="r")
+        dcf = np.load(os.path.join(d, f"dcf_{b}.npy"))
+        ub = np.load(os.path.join(d, f"img_{b}_{e}.npy"))
+        mp = np.load(os.path.join(d, f"map_{c}.npy"))
+        M = coord.shape[0]
+        y = (rng.standard_normal(M, dtype=np.float32) + 1j * rng.standard_normal(M, dtype=np.float32)) \
+            .astype(np.complex64) * np.float32(1e-3)
+        q = np.zeros(M, dtype=np.complex64)
+        g = np.zeros(shape, dtype=np.complex64)
+        coord = np.ascontiguousarray(coord)
+        t0 = time.perf_counter()
+        s32.data_channel(ub, mp, y, q, dcf, coord, g)
+        out["channels"].append({"bin": b, "coil": c, "echo": e, "samples": int(M),
+                                "seconds": time.perf_counter() - t0})
+    if job.get("ew_slab"):
+        B, E = job["bins"], job["echoes"]
+        sh = (B, E, min(EW_SLAB_Z, shape[0]), shape[1], shape[2])
+
+        def rnd():
+            return (rng.standard_normal(sh, dtype=np.float32)
+                    + 1j * rng.standard_normal(sh, dtype=np.float32)).astype(np.complex64)
+        u, ub, g = rnd(), rnd(), rnd()
+        z = np.zeros(sh, dtype=np.complex64)
+        t0 = time.perf_counter()
+        s32.tvme_elementwise(u, ub, g, z, z, z, z, job["lam_m"], job["lam_e"], True)
+        out["ew_slab_s"] = time.perf_counter() - t0
+        out["ew_slab_z"] = sh[2]
+    print(json.dumps(out))
 
 
-def run_cpu_baseline(cfg_name, iterations, verbose):
+def _cpu_env():
     env = dict(os.environ)
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
         env[k] = "1"
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", "--config", cfg_name]
-    log("cpu baseline: timing the float64 oracle on 1 core", verbose)
-    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
-    if r.returncode != 0:
-        log(f"cpu baseline failed: {r.stderr[-2000:]}", True)
-        return None
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    est = d["per_iter"] * iterations
-    sample = (f"float64 oracle, 1 thread: forward+adjoint NUFFT of 1 bin x 1 (coil, echo) channel "
-              f"({d['samples']} samples; {d['t_fwd']:.1f} s + {d['t_adj']:.1f} s) plus one bin of TVME "
-              f"element-wise steps ({d['t_ew_bin']:.2f} s), scaled x{d['K']} channels x{d['bins']} bins "
-              f"= {d['per_iter']:.0f} s/iteration, x{iterations} iterations")
-    out = {"value": est, "unit": "s", "cores": 1, "kind": "port", "sample": sample,
-           "s_per_iteration": d["per_iter"]}
-    # all host cores: the (bin, channel) NUFFTs are independent, so W single-threaded processes
-    # each time the same one-channel sample at once (memory bandwidth shared as in a real run)
-    W = max(1, min(16, len(os.sched_getaffinity(0))))
-    log(f"cpu baseline: {W} concurrent single-threaded oracle processes", verbose)
-    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
-             for _ in range(W)]
-    res = []
-    for pr in procs:
-        so, se = pr.communicate(timeout=1200)
-        if pr.returncode == 0:
-            res.append(json.loads(so.strip().splitlines()[-1]))
-    if len(res) == W:
-        t_ch = sum(x["t_fwd"] + x["t_adj"] for x in res) / W
-        t_ew = sum(x["t_ew_bin"] for x in res) / W
-        per_iter = d["bins"] * (d["K"] * t_ch / W + t_ew / W)
-        out["all_cores"] = {"value": per_iter * iterations, "unit": "s", "cores": W, "kind": "port",
-                            "s_per_iteration": per_iter,
-                            "sample": f"{W} processes x 1 thread, each timing the 1-channel forward+adjoint "
-                                      f"sample ({t_ch:.1f} s under load) and one bin's element-wise steps; "
-                                      f"{d['bins']} bins x {d['K']} channels spread over {W} cores"}
-    return out
+    return env
 
+
+def _spawn_worker(job, tmpdir, name):
+    path = os.path.join(tmpdir, f"{name}.json")
+    with open(path, "w") as f:
+        json.dump(job, f)
+    return subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", path],
+                            stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=_cpu_env())
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def run_cpu_baseline(acq, cfg, n_plain, verbose, deadline):
+    """The single-precision CPU path on one channel of every bin (1 pinned core, bounded by
+    CPU_BUDGET_S) and on W concurrent channels (all host cores), extrapolated to the plain solve."""
+    import numpy as np
+    B, E, C = cfg.bins, cfg.echoes, cfg.coils
+    K = E * C
+    Z = cfg.img_shape[0]
+    cores = sorted(os.sched_getaffinity(0))
+    W = max(1, min(16, len(cores)))
+    tmp = tempfile.mkdtemp(prefix="tr_cpu_")
+    chans = [{"bin": b, "coil": (7 * b + 3) % C, "echo": b % E} for b in range(B)]
+    par = [{"bin": w % B, "coil": (5 * w + 1) % C, "echo": (w // B + w) % E} for w in range(W)]
+    try:
+        for b in range(B):
+            np.save(os.path.join(tmp, f"coord_{b}.npy"), acq.coord[acq.bin_index(b)].reshape(-1, 3))
+            np.save(os.path.join(tmp, f"dcf_{b}.npy"), acq.dcf[acq.bin_index(b)].reshape(-1))
+        for ch in chans + par:
+            p = os.path.join(tmp, f"img_{ch['bin']}_{ch['echo']}.npy")
+            if not os.path.exists(p):
+                np.save(p, acq.truth[ch["bin"], ch["echo"]])
+            p = os.path.join(tmp, f"map_{ch['coil']}.npy")
+            if not os.path.exists(p):
+                np.save(p, acq.mps[ch["coil"]])
+        base = {"dir": tmp, "shape": list(cfg.img_shape), "bins": B, "echoes": E,
+                "lam_m": cfg.lambda_motion, "lam_e": cfg.lambda_echo}
+        log(f"cpu baseline: single-precision CPU path, 1 pinned core, one channel of each of {B} bins", verbose)
+        pr = _spawn_worker(dict(base, channels=chans, core=cores[0], ew_slab=True, budget_s=CPU_BUDGET_S),
+                           tmp, "single")
+        so, se = pr.communicate(timeout=CPU_BUDGET_S + 240)
+        if pr.returncode != 0:
+            log(f"cpu baseline failed: {se[-2000:]}", True)
+            return None
+        d1 = json.loads(so.strip().splitlines()[-1])
+        done = d1["channels"]
+        t_ch = sum(x["seconds"] for x in done) / len(done)
+        ew_iter = d1["ew_slab_s"] * Z / d1["ew_slab_z"]
+        per_iter = B * K * t_ch + ew_iter
+        bins_done = sorted({x["bin"] for x in done})
+        out = {"value": per_iter * n_plain, "unit": "s", "cores": 1, "kind": "port",
+               "s_per_iteration": per_iter, "iterations": n_plain, "cpu_model": _cpu_model(),
+               "sample": (f"single-precision NumPy restatement of the reference's --device -1 path "
+                          f"(oracle/sigpy32.py, numpy.fft, 1 thread pinned to one core): the data term of one "
+                          f"(coil, echo) channel in each of bins {bins_done} "
+                          f"({', '.join('%.2f' % x['seconds'] for x in done)} s; forward NUFFT, dual prox, "
+                          f"weighted adjoint, coil accumulation over {done[0]['samples']} samples) and the TVME "
+                          f"element-wise steps on a {d1['ew_slab_z']}-plane z-slab ({d1['ew_slab_s']:.2f} s); "
+                          f"per iteration = {B} bins x {K} channels x {t_ch:.2f} s + {ew_iter:.1f} s = "
+                          f"{per_iter:.0f} s, x{n_plain} iterations of the plain solve "
+                          f"(extrapolation factor {B * K * n_plain / len(done):.0f} on the sampled channel time)")}
+        if time.perf_counter() - T_PROCESS0 + 2.5 * t_ch + 10 > deadline:
+            log("cpu baseline: no time left for the all-cores round", verbose)
+            return out
+        log(f"cpu baseline: {W} concurrent single-threaded workers, one channel each", verbose)
+        start_at = time.time() + 6.0
+        procs = [_spawn_worker(dict(base, channels=[par[w]], core=None, ew_slab=False, start_at=start_at,
+                                    seed=w), tmp, f"par{w}") for w in range(W)]
+        res = []
+        for p in procs:
+            so, se = p.communicate(timeout=600)
+            if p.returncode == 0:
+                res.append(json.loads(so.strip().splitlines()[-1]))
+        if len(res) == W and all(r["channels"] for r in res):
+            t_round = max(r["channels"][0]["seconds"] for r in res)
+            per_iter_all = B * K / W * t_round + ew_iter / W
+            out["all_cores"] = {"value": per_iter_all * n_plain, "unit": "s", "cores": W, "kind": "port",
+                                "s_per_iteration": per_iter_all,
+                                "sample": (f"{W} single-threaded processes started together, each running the "
+                                           f"data term of a different (bin, coil, echo) channel; the round took "
+                                           f"{t_round:.2f} s (slowest worker); per iteration = {B * K} channels "
+                                           f"/ {W} x {t_round:.2f} s + the element-wise steps / {W}")}
+        return out
+    finally:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+# ---------------------------------------------------------------------------------------------
+# hardware counters -> profiles/traffic.json
+# ---------------------------------------------------------------------------------------------
 
 # Kernel symbols -> the logical kernel names used in the JSON line.  The line
 # passes share one symbol per length; forward and inverse passes are told
@@ -235,7 +386,7 @@ def _launch_counters(path, counter):
     return launches, copy_counted
 
 
-def write_traffic(fetch_csv, write_csv, copy_mib, out_path):
+def write_traffic(fetch_csv, write_csv, copy_mib, out_path, command=None):
     """HBM bytes per launch of every logical kernel from FETCH_SIZE / WRITE_SIZE, each
     scaled by copy bytes / what the counter reads for the calibration copy."""
     import statistics
@@ -255,11 +406,15 @@ def write_traffic(fetch_csv, write_csv, copy_mib, out_path):
                          "launches_sampled": len(fetch.get(name, []))}
     out = {"calibration": {"copy_bytes": copy_bytes, "fetch_factor": f_factor, "write_factor": w_factor,
                            "note": "factor = copy_bytes / counter value of one device copy of copy_bytes"},
-           "kernels": kernels}
+           "command": command, "kernels": kernels}
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     return out
 
+
+# ---------------------------------------------------------------------------------------------
+# other configurations
+# ---------------------------------------------------------------------------------------------
 
 def bench_batched(args, verbose=True):
     """Twelve uneven TVM instances: sequential eager solves vs one HIP-graph batch."""
@@ -343,26 +498,34 @@ def bench_batched(args, verbose=True):
     return 0
 
 
-def tvm48_oracle_iteration():
-    """One full float64-oracle TVM PDHG iteration at 48x64x64 (all bins, coils, echoes), 1 thread."""
+def _tvm48_cpu_iteration():
+    """One full single-precision CPU TVM iteration at 48x64x64 (every bin, coil and echo), 1 thread."""
     import numpy as np
 
-    from oracle import nufft as on
     from oracle import pdhg as op
+    from oracle import sigpy32 as s32
     from toporecon_hip import synth
     cfg = synth.config_small_tvm()
     acq = synth.make_acquisition(cfg)
-    ksp = synth.simulate_kspace_numpy(acq, lambda x, c: on.nufft(x.astype(np.complex128), c))
-    prob = op.bin_problem(ksp, acq.coord, acq.dcf, acq.mps, acq.resp, acq.bin_edges)
+    ksp = synth.simulate_kspace_numpy(acq, lambda x, c: np.stack(
+        [np.stack([s32.nufft(x[e, k], c) for k in range(x.shape[1])]) for e in range(x.shape[0])]))
+    prob = op.bin_problem(ksp, acq.coord, acq.dcf, acq.mps.astype(np.complex64), acq.resp, acq.bin_edges)
+    shape = (prob.B, prob.E) + prob.img_shape
+    st = {"u": np.zeros(shape, np.complex64), "ubar": np.zeros(shape, np.complex64),
+          "p_m": np.zeros(shape, np.complex64),
+          "q": [np.zeros(k.shape, np.complex64) for k in prob.ksp]}
+    prob.ksp = [k.astype(np.complex64) for k in prob.ksp]
+    prob.dcf = [d.astype(np.float32) for d in prob.dcf]
     t0 = time.perf_counter()
-    op.run(prob, "tvm", lam_m=cfg.lambda_motion, l2_coupling=cfg.l2_coupling, max_iter=1, tol=0.0)
+    s32.tvm_iteration(prob, st, cfg.lambda_motion, cfg.l2_coupling)
     print(json.dumps({"per_iter": time.perf_counter() - t0, "bins": prob.B, "K": prob.E * prob.C,
                       "samples": [int(k.shape[-1]) for k in prob.ksp]}))
 
 
 def bench_tvm48(args, verbose=True):
     """Plain TVM to tol 1e-3 at 48x64x64 (8 coils, 2 echoes, 4 bins, 1536x256 cone samples) on the GPU,
-    beside one full float64-oracle iteration on one host core, scaled to the GPU's iteration count."""
+    beside one full single-precision CPU iteration on one host core, scaled to the GPU's
+    iteration count."""
     import torch
 
     from toporecon_hip import synth
@@ -393,16 +556,21 @@ def bench_tvm48(args, verbose=True):
         env = dict(os.environ)
         for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
             env[k] = "1"
-        log("cpu baseline: one float64 oracle iteration on 1 core", verbose)
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", "--config", "tvm48"],
+        log("cpu baseline: one single-precision CPU iteration on 1 pinned core", verbose)
+        core = sorted(os.sched_getaffinity(0))[0]
+        r = subprocess.run([sys.executable, "-c", "import os, sys; os.sched_setaffinity(0, {%d}); "
+                            "sys.argv = ['bench.py', '--cpu-baseline-worker', 'tvm48']; "
+                            "sys.path.insert(0, %r); import bench; bench._tvm48_cpu_iteration()" % (core, REPO)],
                            capture_output=True, text=True, env=env, timeout=900)
         if r.returncode == 0:
             d = json.loads(r.stdout.strip().splitlines()[-1])
             cpu = {"value": d["per_iter"] * n_it, "unit": "s", "cores": 1, "kind": "port",
                    "s_per_iteration": d["per_iter"],
-                   "sample": f"one full float64-oracle TVM iteration ({d['bins']} bins x {d['K']} channels, "
-                             f"{sum(d['samples'])} samples) = {d['per_iter']:.1f} s on 1 thread, "
-                             f"x{n_it} iterations"}
+                   "cpu_model": _cpu_model(),
+                   "sample": f"one full TVM iteration of the single-precision NumPy restatement of the "
+                             f"reference's --device -1 path (oracle/sigpy32.py; {d['bins']} bins x {d['K']} "
+                             f"channels, {sum(d['samples'])} samples) = {d['per_iter']:.1f} s on 1 pinned "
+                             f"thread, x{n_it} iterations of the GPU solve"}
         else:
             log(f"cpu baseline failed: {r.stderr[-2000:]}", True)
     line = {
@@ -500,42 +668,42 @@ def bench_cnn(args, verbose=True):
     return 0
 
 
-def main(argv=None):
-    args = parse_args(argv)
-    if args.config == "cnn":
-        return bench_cnn(args)
-    if args.cpu_baseline_worker and args.config == "tvm48":
-        tvm48_oracle_iteration()
-        return 0
-    if args.config == "tvm48":
-        if args.gpus != 1:
-            print("bench.py measures one GPU (run with --gpus 1)", file=sys.stderr)
-            return 2
-        return bench_tvm48(args)
-    if args.config == "batched":
-        if args.gpus != 1:
-            print("bench.py measures one GPU (run with --gpus 1)", file=sys.stderr)
-            return 2
-        return bench_batched(args)
-    if args.traffic_from:
-        out = write_traffic(args.traffic_from[0], args.traffic_from[1], args.copy_calibration_mib or 2048,
-                            TRAFFIC_FILE)
-        print(json.dumps(out["calibration"]))
-        return 0
-    if args.cpu_baseline_worker:
-        cpu_baseline_worker(args.config)
-        return 0
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the HIP path has no multi-GPU sharding "
-              "(run with --gpus 1)", file=sys.stderr)
-        return 2
-    verbose = True
-    import numpy as np
+# ---------------------------------------------------------------------------------------------
+# the headline: TVME at 80x192x192, learned warm start vs plain
+# ---------------------------------------------------------------------------------------------
+
+def _set_params(solver, params, verbose):
+    tuned = json.loads(params)
+    for k in ("interp_xcd_chunk", "interp_kgroups", "spread_xcd_chunk", "lines_xcd_lines"):
+        if k in tuned:
+            solver.plan.set_param(k, int(tuned[k]))
+    log(f"launch parameters set: {tuned}", verbose)
+    return tuned
+
+
+def _kernel_table(prof, cfg, samples, grid_shape):
+    """Per logical kernel: launches, average HIP-event duration, GB/s on the must-move bytes the
+    library credits, and GB/s on the per-unit byte model (launches spread evenly over the bins)."""
+    E, C, B = cfg.echoes, cfg.coils, cfg.bins
+    N = math.prod(cfg.img_shape)
+    G = math.prod(grid_shape)
+    M_avg = sum(samples) / max(len(samples), 1)
+    out = {}
+    for name, (ms, n, by) in prof.items():
+        model = model_bytes_per_launch(name, M_avg, E, C, N, G, B)
+        out[name] = {"launches": n, "ms_avg": ms / n, "ms_total": ms,
+                     "must_move_bytes_per_launch": by / n,
+                     "gbps_must_move": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                     "model_bytes_per_launch": model,
+                     "gbps_model": model * n / (ms * 1e-3) / 1e9 if (model and ms > 0) else None}
+    return out
+
+
+def bench_full(args, verbose=True):
     import torch
 
-    from toporecon_hip import _lib, synth
+    from toporecon_hip import _lib, objective, synth
     from toporecon_hip.algorithms import TvmeReconstructor
-
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
@@ -545,6 +713,8 @@ def main(argv=None):
         cfg.max_iter = args.max_iter
     if args.tol is not None:
         cfg.tol = args.tol
+    plan_w = work_plan(args)
+    log(f"work plan: {plan_w}", verbose)
     t0 = time.perf_counter()
     acq = synth.make_acquisition(cfg)
     log(f"synthetic acquisition {cfg.img_shape} C={cfg.coils} E={cfg.echoes} B={cfg.bins} "
@@ -569,17 +739,13 @@ def main(argv=None):
         del src, dst
     t0 = time.perf_counter()
     if args.params:
-        tuned = json.loads(args.params)
-        for k in ("interp_xcd_chunk", "interp_kgroups", "spread_xcd_chunk", "lines_xcd_lines"):
-            if k in tuned:
-                solver.plan.set_param(k, int(tuned[k]))
-        solver.set_bin_streams(int(tuned.get("bin_streams", 1)))
-        log(f"launch parameters set: {tuned}", verbose)
+        tuned = _set_params(solver, args.params, verbose)
     else:
-        tuned = solver.autotune()
-        log(f"autotune in {time.perf_counter() - t0:.1f} s: {tuned}", verbose)
+        # bins stay on one stream: the per-kernel HIP events then time one kernel each
+        tuned = solver.autotune(bin_streams=False)
+        log(f"autotune in {time.perf_counter() - t0:.1f} s: "
+            f"{ {k: tuned.get(k) for k in ('interp_xcd_chunk', 'interp_kgroups', 'spread_xcd_chunk')} }", verbose)
 
-    from toporecon_hip import objective
     from toporecon_hip.learned import DEFAULT_CG_STEPS, WarmStartNet, default_weights_path, warm_start
     cg_steps = DEFAULT_CG_STEPS if args.cg_steps is None else args.cg_steps
     net = None
@@ -590,137 +756,170 @@ def main(argv=None):
     else:
         log("no trained warm-start weights: the learned leg is skipped (plain PDHG only)", verbose)
 
-    # warmup: short solves through both paths (kernels, plans and caches warm)
+    def learned_solve(progress=0):
+        u0 = warm_start(solver, net, cg_steps=cg_steps)
+        u = solver.run(init=u0, dual_init="residual", progress_every=progress)
+        del u0
+        return u
+
+    # warmup: learned solves capped at WARMUP_ITERS PDHG iterations (every kernel and code path runs)
     full_iter = solver.max_iter
-    solver.max_iter = min(full_iter, 5)
+    solver.max_iter = min(full_iter, WARMUP_ITERS)
     for w in range(args.warmup):
-        solver.run()
         if net is not None:
-            solver.run(init=warm_start(solver, net, cg_steps=cg_steps), dual_init="residual")
+            learned_solve()
+        else:
+            solver.run()
         torch.cuda.synchronize(dev)
-        log(f"warmup {w} done", verbose)
+        log(f"warmup step {w + 1}/{args.warmup} done", verbose)
     solver.max_iter = full_iter
 
-    # plain PDHG from zeros
+    # plain PDHG from zeros: the denominator, timed once (--legs plain: the timed steps)
+    n_plain_solves = plan_w["plain_solves"]
+    timed_events = not args.no_timed_events
+    if args.legs == "plain" and timed_events:
+        _lib.profile_enable(True)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    iters = []
-    for k in range(args.steps):
+    done_plain = 0
+    for k in range(n_plain_solves):
         u_plain = solver.run(progress_every=25)
-        iters.append(solver.iterations)
+        done_plain += 1
+        if args.legs == "plain" and timed_events:
+            _lib.profile_read()  # drain this solve's events
+        el = time.perf_counter() - t0
+        if time.perf_counter() - T_PROCESS0 + el / done_plain > args.budget_s:
+            break
     torch.cuda.synchronize(dev)
-    elapsed = time.perf_counter() - t0
-    t_plain = elapsed / args.steps
-    n_it = iters[-1]
+    t_plain = (time.perf_counter() - t0) / done_plain
+    n_plain = solver.iterations
     u_plain = u_plain.clone()
     phi_plain = objective.objective(solver, u_plain)
     del u_plain
-    log(f"plain: {args.steps} solve(s), {n_it} iterations, {t_plain:.2f} s per solve, objective {phi_plain:.6e}",
-        verbose)
+    log(f"plain: {done_plain} solve(s), {n_plain} iterations, {t_plain:.2f} s per solve, "
+        f"objective {phi_plain:.6e}", verbose)
 
-    # learned warm start: dcf adjoint -> CNN (MFMA, bf16) -> data-fitted scale -> CG steps -> PDHG
-    def timed_start(use_net, k_cg, steps):
-        torch.cuda.synchronize(dev)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            ta = time.perf_counter()
-            u0 = warm_start(solver, net if use_net else None, cg_steps=k_cg)
-            torch.cuda.synchronize(dev)
-            t_start = time.perf_counter() - ta
-            u_warm = solver.run(init=u0, dual_init="residual", progress_every=25)
-        torch.cuda.synchronize(dev)
-        t_warm = (time.perf_counter() - t0) / steps
-        phi_warm = objective.objective(solver, u_warm)
-        rel = (phi_warm - phi_plain) / phi_plain
-        del u0, u_warm
-        return {"start": ("cnn" if use_net else "zero") + f"+{k_cg}cg", "cg_steps": k_cg,
-                "time_to_tol_s": t_warm, "iterations": solver.iterations, "warm_start_s": t_start,
-                "objective": phi_warm, "objective_plain": phi_plain, "objective_rel_diff": rel,
-                "within_0.1pct": abs(rel) <= 1e-3, "not_worse_than_plain_0.1pct": rel <= 1e-3,
-                "speedup_vs_plain": t_plain / t_warm}
-
+    # learned leg: the timed steps
     warm = None
-    variants = []
+    steps_done = 0
+    t_learned = None
     if net is not None:
-        warm = timed_start(True, cg_steps, args.steps)
-        warm.update(weights=os.path.basename(default_weights_path()), trained_on=net.meta)
-        log(f"warm start {warm['start']}: {warm['iterations']} iterations, {warm['time_to_tol_s']:.2f} s "
-            f"(start {warm['warm_start_s']:.2f} s), objective {warm['objective']:.6e} "
-            f"({warm['objective_rel_diff']:+.2e} vs plain), speedup {warm['speedup_vs_plain']:.2f}x", verbose)
-        for v in filter(None, (x.strip() for x in args.variants.split(","))):
-            kind, k_cg = v.split("+")
-            r = timed_start(kind == "cnn", int(k_cg), 1)
-            variants.append(r)
-            log(f"variant {r['start']}: {r['iterations']} iterations, {r['time_to_tol_s']:.2f} s, "
-                f"objective {r['objective_rel_diff']:+.2e} vs plain, speedup {r['speedup_vs_plain']:.2f}x", verbose)
-    # step-size check: float64 power iteration for ||K|| (outside the timed legs)
-    knorm = None
-    if args.legs == "all":
+        if timed_events:
+            _lib.profile_enable(True)
+        torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        knorm = solver.operator_norm(iters=20)
+        t_last = 0.0
+        for k in range(args.steps):
+            if k > 0 and time.perf_counter() - T_PROCESS0 + 1.2 * t_last > args.budget_s:
+                log(f"wall-clock guard: stopping after {k} of {args.steps} timed steps", True)
+                break
+            ta = time.perf_counter()
+            u_warm = learned_solve(progress=25 if k == 0 else 0)
+            torch.cuda.synchronize(dev)
+            t_last = time.perf_counter() - ta
+            steps_done += 1
+            if timed_events:
+                _lib.profile_read()  # drain this step's events (keeps the event pool small)
+            log(f"timed step {k + 1}/{args.steps}: {t_last:.3f} s, {solver.iterations} iterations", verbose)
+        torch.cuda.synchronize(dev)
+        t_learned = (time.perf_counter() - t0) / steps_done
+        n_learned = solver.iterations
+        phi_warm = objective.objective(solver, u_warm)
+        del u_warm
+        rel = (phi_warm - phi_plain) / phi_plain
+        warm = {"start": f"cnn+{cg_steps}cg", "cg_steps": cg_steps, "time_to_tol_s": t_learned,
+                "iterations": n_learned, "objective": phi_warm, "objective_rel_diff": rel,
+                "within_0.1pct": abs(rel) <= 1e-3, "speedup_vs_plain": t_plain / t_learned,
+                "weights": os.path.basename(default_weights_path()), "trained_on": net.meta}
+        log(f"learned: {steps_done} step(s), {n_learned} iterations, {t_learned:.3f} s per solve, objective "
+            f"{phi_warm:.6e} ({rel:+.2e} vs plain), speedup {t_plain / t_learned:.2f}x", verbose)
+    prof = _lib.profile_read() if timed_events else {}
+    _lib.profile_enable(False)
+
+    variants = []
+    for v in filter(None, (x.strip() for x in args.variants.split(","))):
+        if net is None:
+            break
+        kind, k_cg = v.split("+")
+        torch.cuda.synchronize(dev)
+        ta = time.perf_counter()
+        u0 = warm_start(solver, net if kind == "cnn" else None, cg_steps=int(k_cg))
+        u_v = solver.run(init=u0, dual_init="residual")
+        torch.cuda.synchronize(dev)
+        tv = time.perf_counter() - ta
+        phi = objective.objective(solver, u_v)
+        del u0, u_v
+        variants.append({"start": f"{kind}+{k_cg}cg", "time_to_tol_s": tv, "iterations": solver.iterations,
+                         "objective_rel_diff": (phi - phi_plain) / phi_plain, "speedup_vs_plain": t_plain / tv})
+        log(f"variant {v}: {solver.iterations} iterations, {tv:.2f} s, objective "
+            f"{variants[-1]['objective_rel_diff']:+.2e} vs plain", verbose)
+
+    knorm = None
+    if plan_w["power_iterations"]:
+        t0 = time.perf_counter()
+        knorm = solver.operator_norm(iters=plan_w["power_iterations"])
         knorm = {k: knorm[k] for k in ("K_norm", "sigma_tau_K_norm_sq", "iterations")}
         log(f"power iteration: ||K|| = {knorm['K_norm']:.4f}, sigma tau ||K||^2 = "
             f"{knorm['sigma_tau_K_norm_sq']:.4f} ({time.perf_counter() - t0:.1f} s)", verbose)
+
     use_warm = warm is not None and warm["within_0.1pct"]
-    t_step = warm["time_to_tol_s"] if use_warm else t_plain
+    if args.legs == "plain":
+        t_step, steps_done = t_plain, done_plain
+    else:
+        t_step = warm["time_to_tol_s"] if use_warm else t_plain
 
-    final_tol = solver.history[-1] if solver.history else None
-    # per-kernel times: a short serial pass (bins on one stream), so that no two kernels overlap
-    # and each HIP-event interval is one kernel's own duration
-    streams_used = len(getattr(solver, "_bstreams", None) or [None])
-    solver.set_bin_streams(1)
-    solver.max_iter, prof_iters = min(3, full_iter), min(3, full_iter)
-    solver.tol = 0.0
-    _lib.profile_enable(True)
-    solver.run()
-    torch.cuda.synchronize(dev)
-    prof = _lib.profile_read()
-    _lib.profile_enable(False)
-    solver.max_iter, solver.tol = full_iter, cfg.tol
-    solver.set_bin_streams(streams_used)
-
-    kernels = {}
-    for name, (ms, n, by) in prof.items():
-        kernels[name] = {"launches": n, "ms_avg": ms / n, "ms_total": ms,
-                         "gbps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
-                         "bytes_per_launch": by / n}
-    dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
-    dk = kernels[dom]
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("kernels", {}).get(dom, {}).get("bytes")
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["gbps"], "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": dk["gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
-                "algorithmic_bytes_per_launch": dk["bytes_per_launch"], "avg_launch_ms": dk["ms_avg"]}
-    # the two gathers take about the same time at this size, so the dominant one can flip between
-    # runs: both are reported
-    gathers = {}
-    all_traffic = {}
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            all_traffic = json.load(f).get("kernels", {})
-    for k in ("interp", "spread"):
-        if k in kernels and kernels[k]["gbps"]:
-            gathers[k] = {"achieved": kernels[k]["gbps"], "frac": kernels[k]["gbps"] / HBM_PEAK_GBPS,
-                          "avg_launch_ms": kernels[k]["ms_avg"],
-                          "algorithmic_bytes_per_launch": kernels[k]["bytes_per_launch"],
-                          "traffic": all_traffic.get(k, {}).get("bytes")}
-    roofline["gathers"] = gathers
-    fwd_b, adj_b = nufft_model_bytes(samples, cfg.echoes, cfg.coils, cfg.img_shape,
-                                      solver.plan.grid_shape)
-    fwd_ms = sum(kernels[k]["ms_total"] for k in ("fwd_x_pad", "fft_y_fwd", "fft_z_fwd", "interp")
-                 if k in kernels)
-    adj_ms = sum(kernels[k]["ms_total"] for k in ("spread", "fft_z_inv", "fft_y_inv", "adj_x_crop")
-                 if k in kernels)
-    total_iters = prof_iters
-    nufft = {"forward_gbps": fwd_b * total_iters / (fwd_ms * 1e-3) / 1e9 if fwd_ms else None,
-             "adjoint_gbps": adj_b * total_iters / (adj_ms * 1e-3) / 1e9 if adj_ms else None,
-             "forward_ms_per_iter": fwd_ms / total_iters, "adjoint_ms_per_iter": adj_ms / total_iters}
+    kernels = _kernel_table(prof, cfg, samples, solver.plan.grid_shape)
+    roofline = None
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+        dk = kernels[dom]
+        traffic, traffic_src = None, None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as f:
+                tj = json.load(f)
+            traffic = tj.get("kernels", {}).get(dom, {}).get("bytes")
+            traffic_src = (f"replayed from profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
+                           f"`{tj.get('command') or 'bench.py'}`, scaled by a calibration copy)")
+        achieved = dk["gbps_model"] if dk["gbps_model"] is not None else dk["gbps_must_move"]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": traffic_src,
+                    "bytes_model": ("per-unit algorithmic bytes x units per launch (interp 8KG + 12M + 24KM, "
+                                    "spread 8KM + 16M + 8KG, pad 8EN + 8CN + 8KG, crop 8KN + 8CN + 8EN, a line "
+                                    "pass 16KG; K = channels, G = oversampled grid points, M = samples of the bin)"),
+                    "algorithmic_bytes_per_launch": dk["model_bytes_per_launch"],
+                    "avg_launch_ms": dk["ms_avg"], "launches": dk["launches"],
+                    "timing": (f"HIP events on the launch stream around every launch in the {steps_done} timed "
+                               f"step(s)"),
+                    "must_move": {"bytes_per_launch": dk["must_move_bytes_per_launch"],
+                                  "achieved": dk["gbps_must_move"],
+                                  "frac": (dk["gbps_must_move"] or 0.0) / HBM_PEAK_GBPS,
+                                  "note": "bytes the kernel must move after k-space support pruning "
+                                          "(touched grid points only)"}}
+        gathers = {}
+        for k in ("interp", "spread"):
+            if k in kernels:
+                gathers[k] = {"avg_launch_ms": kernels[k]["ms_avg"], "achieved": kernels[k]["gbps_model"],
+                              "frac": (kernels[k]["gbps_model"] or 0.0) / HBM_PEAK_GBPS,
+                              "achieved_must_move": kernels[k]["gbps_must_move"]}
+        roofline["gathers"] = gathers
+    fwd_b, adj_b = nufft_model_bytes(samples, cfg.echoes, cfg.coils, cfg.img_shape, solver.plan.grid_shape)
+    nufft = None
+    if kernels:
+        n_fwd = kernels.get("interp", {}).get("launches", 0) / max(cfg.bins, 1)
+        n_adj = kernels.get("spread", {}).get("launches", 0) / max(cfg.bins, 1)
+        fwd_ms = sum(kernels[k]["ms_total"] for k in ("fwd_x_pad", "fft_y_fwd", "fft_z_fwd", "interp")
+                     if k in kernels)
+        adj_ms = sum(kernels[k]["ms_total"] for k in ("spread", "fft_z_inv", "fft_y_inv", "adj_x_crop")
+                     if k in kernels)
+        nufft = {"forward_gbps": fwd_b * n_fwd / (fwd_ms * 1e-3) / 1e9 if fwd_ms else None,
+                 "adjoint_gbps": adj_b * n_adj / (adj_ms * 1e-3) / 1e9 if adj_ms else None,
+                 "forward_ms_all_bins": fwd_ms / max(n_fwd, 1), "adjoint_ms_all_bins": adj_ms / max(n_adj, 1),
+                 "bytes_model": "pad 8EN+8CN+8KG, fft3 16KG, interp 8KG+12M+24KM per bin (forward); "
+                                "spread 8KM+16M+8KG, ifft3 16KG, crop 8KN+8CN+8EN per bin (adjoint)"}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = run_cpu_baseline(args.config, n_it, verbose)
+        cpu = run_cpu_baseline(acq, cfg, n_plain, verbose, deadline=args.budget_s + 100)
 
     shape = "x".join(str(v) for v in cfg.img_shape)
     out = {
@@ -728,7 +927,7 @@ def main(argv=None):
         "value": t_step,
         "unit": "s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps_done,
         "warmup": args.warmup,
         "ms_per_step": t_step * 1e3,
         "higher_is_better": False,
@@ -742,26 +941,53 @@ def main(argv=None):
                                f"{shape}, {cfg.coils} coils, {cfg.echoes} echoes, {cfg.bins} bins, "
                                f"{cfg.interleaves}x{cfg.readouts} cone samples/echo, "
                                f"l2 coupling, lambda {cfg.lambda_motion}",
+                   "step": ("one learned-warm-start solve to tolerance (warm start + PDHG)" if args.legs == "all"
+                            else "one plain PDHG solve to tolerance"),
+                   "steps_requested": args.steps,
                    "start": "learned" if use_warm else "plain",
-                   "plain": {"time_to_tol_s": t_plain, "iterations": n_it, "objective": phi_plain,
-                             "s_per_iteration": t_plain / max(n_it, 1)},
+                   "plain": {"time_to_tol_s": t_plain, "iterations": n_plain, "objective": phi_plain,
+                             "s_per_iteration": t_plain / max(n_plain, 1), "solves_timed": done_plain},
                    "learned_warm_start": warm,
                    "warm_start_variants": variants,
                    "operator_norm": knorm,
-                   "iterations": warm["iterations"] if use_warm else n_it,
-                   "s_per_iteration": t_plain / max(n_it, 1),
+                   "iterations": warm["iterations"] if use_warm else n_plain,
+                   "s_per_iteration": t_plain / max(n_plain, 1),
                    "samples_per_bin": samples, "grid": list(solver.plan.grid_shape),
                    "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "interp_kgroups",
-                                                                "spread_xcd_chunk", "bin_streams")},
-                   "final_tol": final_tol, "kernel_profile": f"serial pass, {prof_iters} iterations, "
-                                                             "bins on one stream"},
+                                                                "spread_xcd_chunk")},
+                   "bin_streams": 1,
+                   "final_tol": solver.history[-1] if solver.history else None,
+                   "run_wall_s": time.perf_counter() - T_PROCESS0},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "nufft": nufft,
         "kernels": kernels,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
     return 0
+
+
+def main(argv=None):
+    args = parse_args(argv)
+    if args.cpu_baseline_worker:
+        _cpu_worker(args.cpu_baseline_worker)
+        return 0
+    if args.traffic_from:
+        out = write_traffic(args.traffic_from[0], args.traffic_from[1], args.copy_calibration_mib or 2048,
+                            TRAFFIC_FILE, command=args.traffic_command)
+        print(json.dumps(out["calibration"]))
+        return 0
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU sharding, so --gpus must be 1 "
+              "(how the path would shard is described in the design notes)", file=sys.stderr)
+        return 2
+    if args.config == "cnn":
+        return bench_cnn(args)
+    if args.config == "tvm48":
+        return bench_tvm48(args)
+    if args.config == "batched":
+        return bench_batched(args)
+    return bench_full(args)
 
 
 if __name__ == "__main__":

@@ -127,6 +127,7 @@ struct tr_plan {
   // the interpolation and never receive a spread contribution, so the line
   // passes skip them.
   int2* col_win = nullptr;
+  std::vector<int2> col_win_host;  // host copy of col_win (tap groups are checked against it)
   int2* ywin = nullptr;  // per x: the y window covering every live column (y, x)
   int prune = 1;
   int64_t live_cols = 0;
@@ -351,6 +352,7 @@ inline GridGeom geom(const tr_plan* p) {
 int ensure_partial(tr_plan* p, int64_t n);
 int ensure_work(tr_plan* p, int64_t K);
 int build_support(tr_plan* p, hipStream_t s);
+int build_tap_groups(tr_plan* p, BinData& b, hipStream_t s);
 void use_slot(tr_plan* p, int slot);
 int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s);
 int count_touched(const tr_plan* p, BinData& b, hipStream_t s);

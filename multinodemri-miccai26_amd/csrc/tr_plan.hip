@@ -119,17 +119,42 @@ static bool box_extend(int P, int E, int p, int n, int g, int& nP, int& nE) {
   return false;
 }
 
+// Every point of the box [P, P + E) (natural positions, mod g per axis) lies
+// in the plan's k-space support (col_win): the pruned forward passes write
+// exactly those points, so a box inside it never reads unwritten workspace.
+static bool box_in_support(const int P[3], const int E[3], const int gdim[3], const std::vector<int2>& win) {
+  for (int dy = 0; dy < E[1]; ++dy) {
+    const int y = (P[1] + dy) % gdim[1];
+    for (int dx = 0; dx < E[2]; ++dx) {
+      const int2 w = win[(size_t)y * gdim[2] + (P[2] + dx) % gdim[2]];
+      if (w.y <= 0) return false;
+      for (int dz = 0; dz < E[0]; ++dz) {
+        int d = (P[0] + dz) % gdim[0] - w.x;
+        if (d < 0) d += gdim[0];
+        if (d >= w.y) return false;
+      }
+    }
+  }
+  return true;
+}
+
 // Tap groups of the grouped interpolation: runs of at most kTapGroup
 // consecutive sorted samples whose taps fit in one box of at most 5 positions
-// per axis (samples with no taps stay alone), greedily in sorted order; and
-// b.tapg, each member's record rewritten to its group's box.  Shifting a
-// member's weights into the box keeps the order of its nonzero terms, and the
-// added terms have weight 0, so the gathered sums equal the per-sample ones.
-static int build_tap_groups(BinData& b, int64_t M, int3 g3, hipStream_t s) {
+// per axis that lies inside the plan's k-space support (samples with no taps
+// stay alone), greedily in sorted order; and b.tapg, each member's record
+// rewritten to its group's box.  Shifting a member's weights into the box
+// keeps the order of its nonzero terms, and the added terms have weight 0, so
+// the gathered sums equal the per-sample ones; the support test keeps every
+// box point one that the pruned forward passes write (0 * stale workspace
+// could be NaN otherwise).
+int build_tap_groups(tr_plan* p, BinData& b, hipStream_t s) {
+  const int64_t M = b.M;
+  if (M == 0 || b.grp) return kOk;
   std::vector<TapRec> h((size_t)M);
   TR_HIP(hipMemcpyAsync(h.data(), b.tap, sizeof(TapRec) * M, hipMemcpyDeviceToHost, s));
   TR_HIP(hipStreamSynchronize(s));
-  const int gdim[3] = {g3.x, g3.y, g3.z};
+  const int gdim[3] = {p->g[0], p->g[1], p->g[2]};
+  const std::vector<int2>& win = p->col_win_host;
   auto ntaps = [](const TapRec& r, int ax) { return (r.n >> (4 * ax)) & 15; };
   std::vector<int> starts;
   starts.reserve((size_t)M / 2 + 2);
@@ -151,7 +176,7 @@ static int build_tap_groups(BinData& b, int64_t M, int3 g3, hipStream_t s) {
         const int n = ntaps(h[k], ax);
         fits = n > 0 && box_extend(P[ax], E[ax], h[k].p0[ax], n, gdim[ax], nP[ax], nE[ax]);
       }
-      if (!fits) break;
+      if (!fits || (!win.empty() && !box_in_support(nP, nE, gdim, win))) break;
       for (int ax = 0; ax < 3; ++ax) {
         P[ax] = nP[ax];
         E[ax] = nE[ax];
@@ -398,7 +423,6 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   TR_HIP(hipMemsetAsync(cnt + M, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_tap_records, dim3(grid1d(M, 256)), dim3(256), 0, s, b.kp, M, g3, p->beta, b.tap, cnt);
   TR_LAUNCH_CHECK();
-  if (int r = build_tap_groups(b, M, g3, s)) return r;
   size_t scan_bytes = 0;
   TR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, (int)(M + 1), s));
   void* scan_tmp = nullptr;
@@ -552,6 +576,7 @@ int build_support(tr_plan* p, hipStream_t s) {
   TR_HIP(hipMemcpy(p->ywin, yw.data(), sizeof(int2) * gx, hipMemcpyHostToDevice));
   TR_HIP(hipMalloc(&p->col_win, sizeof(int2) * ncol));
   TR_HIP(hipMemcpy(p->col_win, win.data(), sizeof(int2) * ncol, hipMemcpyHostToDevice));
+  p->col_win_host = std::move(win);
   return kOk;
 }
 
@@ -665,7 +690,9 @@ int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float
       return r;
     }
   }
-  if (const int r = build_support(p, s)) {
+  int r = build_support(p, s);
+  for (int b = 0; b < nbins && !r; ++b) r = build_tap_groups(p, p->bins[b], s);
+  if (r) {
     std::string msg = last_error();
     free_plan(p);
     set_error(msg);

@@ -223,20 +223,32 @@ def run_main(algorithm: str, argv=None) -> int:
         img = rec.run().cpu().numpy()
     logging.info("Reconstruction done in %.2f seconds (%d iterations).", time.perf_counter() - t0,
                  rec.iterations)
+    crop = data["orig_img_shape"] if args.crop_to_orig and not np.allclose(data["fov"], 1.0) else None
+    write_output(img, args.output_dir, args.img_file, _manifest(args, algorithm, data["E_total"], data["prepared"]),
+                 crop_zyx=crop)
+    logging.info("Main done in %.2f seconds.", time.perf_counter() - t_main)
+    return 0
+
+
+def write_output(img, output_dir, stem, manifest, crop_zyx=None):
+    """The reconstruction shard of one process that owns every bin and echo: ubar (B, E, Z, Y, X)
+    as the 9-D (B, 1, 1, E, 1, 1, Z, Y, X) complex64 CFL ``{stem}_e0-{E-1}_m0-{B-1}`` plus
+    ``run_manifest.json`` (tvm.py:1020-1083, tvme.py:984-1048, tvmw.py:1170-1234); the reference's
+    ``toporecon stitch`` assembles it unchanged.  Returns the shard's base path."""
+    img = np.asarray(img)
     Bn, E = img.shape[:2]
     Z, Y, X = img.shape[-3:]
     out = np.zeros((Bn, 1, 1, E, 1, 1, Z, Y, X), dtype=np.complex64)
     out[:, 0, 0, :, 0, 0] = img
-    if args.crop_to_orig and not np.allclose(data["fov"], 1.0):
-        out = center_crop_zyx(out, data["orig_img_shape"])
-    os.makedirs(args.output_dir, exist_ok=True)
-    E_total = data["E_total"]
-    cfl.write_cfl(os.path.join(args.output_dir, f"{args.img_file}_e0-{E_total - 1}_m0-{B - 1}"), out)
-    with open(os.path.join(args.output_dir, "run_manifest.json"), "w", encoding="utf-8") as f:
-        json.dump(_manifest(args, algorithm, E_total, data["prepared"]), f, indent=2, sort_keys=True)
+    if crop_zyx is not None:
+        out = center_crop_zyx(out, crop_zyx)
+    os.makedirs(output_dir, exist_ok=True)
+    base = os.path.join(output_dir, f"{stem}_e0-{E - 1}_m0-{Bn - 1}")
+    cfl.write_cfl(base, out)
+    with open(os.path.join(output_dir, "run_manifest.json"), "w", encoding="utf-8") as f:
+        json.dump(manifest, f, indent=2, sort_keys=True)
         f.write("\n")
-    logging.info("Main done in %.2f seconds.", time.perf_counter() - t_main)
-    return 0
+    return base
 
 
 def tvm_main(argv=None) -> int:
@@ -252,34 +264,21 @@ def tvmw_main(argv=None) -> int:
 
 
 def cli_main(argv=None) -> int:
-    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS``, ``stitch INPUT_DIR`` and
-    ``resp INPUT --work-dir DIR`` as in src/toporecon/cli.py:75-183."""
+    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS`` and ``resp INPUT --work-dir DIR`` as in
+    src/toporecon/cli.py:75-183.  The shards are the reference's format, so its own ``toporecon
+    stitch`` assembles them."""
     p = argparse.ArgumentParser(prog="toporecon_hip")
     p.add_argument("--verbose", action="store_true")
     sub = p.add_subparsers(dest="command", required=True)
     r = sub.add_parser("reconstruct")
     r.add_argument("--algorithm", choices=ALGORITHMS, required=True)
     r.add_argument("algorithm_arguments", nargs=argparse.REMAINDER)
-    s = sub.add_parser("stitch", help="Assemble reconstruction shards into one CFL/HDR image.")
-    s.add_argument("input_dir", type=Path)
-    s.add_argument("--input-prefix", "--stem", dest="stem",
-                   help="Shard prefix; default: output_stem from run_manifest.json.")
-    dest = s.add_mutually_exclusive_group()
-    dest.add_argument("--output-name", default="imout", help="Final filename inside INPUT_DIR; default: imout.")
-    dest.add_argument("--output", type=Path, help="Full path for the final image.")
-    s.add_argument("--overwrite", action="store_true")
     rp = sub.add_parser("resp", help="Estimate resp.hdr/.cfl in a separate work directory.")
     rp.add_argument("input", type=Path)
     rp.add_argument("--work-dir", type=Path, required=True)
     args = p.parse_args(argv)
     logging.basicConfig(level=logging.DEBUG if args.verbose else logging.INFO,
                         format="%(levelname)s: %(message)s")
-    if args.command == "stitch":
-        from .postprocessing import stitch
-        out = stitch(args.input_dir, stem=args.stem, output_name=args.output_name, output=args.output,
-                     overwrite=args.overwrite)
-        print(f"Wrote {out}.hdr and {out}.cfl")
-        return 0
     if args.command == "resp":
         from .preprocessing import estimate_respiratory_from_dataset
         args.work_dir.mkdir(parents=True, exist_ok=True)

@@ -210,3 +210,42 @@ def test_partial_support_forward_adjoint(torch_mod, shape, frac):
     plan.set_param("prune", 1)
     assert np.array_equal(fwd.view(np.uint64), fwd0.view(np.uint64))
     assert np.array_equal(adj.view(np.uint64), adj0.view(np.uint64))
+
+
+def test_grouped_interp_never_reads_unwritten_workspace(torch_mod):
+    """With support pruning the forward passes leave grid points outside the k-space support
+    unwritten; the grouped interpolation's 5^3 boxes must stay inside the support, or weight-0
+    terms would read stale workspace (0 * NaN = NaN).  The workspace is filled with NaN first."""
+    import torch
+
+    from toporecon_hip import _lib, synth
+    from toporecon_hip.nufft_backend import NufftPlan
+    print("grouped interpolation vs stale workspace", flush=True)
+    torch = torch_mod
+    shape = (16, 24, 24)
+    coord = synth.cone_trajectory(300, 64, shape).reshape(-1, 3)
+    dev = torch.device("cuda", 0)
+    plan = NufftPlan(shape, [torch.from_numpy(coord).to(dev)], dev)
+    G = int(np.prod(plan.grid_shape))
+    assert plan.get_param("support_points") < G
+    E, C = 3, 32
+    K = E * C
+    rng = np.random.default_rng(7)
+    img = torch.from_numpy(_cplx(rng, (E, *shape))).to(dev)
+    maps = torch.from_numpy(_cplx(rng, (C, *shape))).to(dev)
+    nan_grid = torch.full((2 * G * K,), float("nan"), dtype=torch.float32, device=dev).view(torch.complex64)
+    out_img = torch.empty((E, *shape), dtype=torch.complex64, device=dev)
+    s = _lib.stream_handle(dev)
+    M = plan.samples(0)
+    plan.set_param("interp_kgroups", 3)
+    ys = []
+    for prune in (1, 0):
+        plan.set_param("prune", prune)
+        _lib.call("tr_nufft_adjoint_grid", plan.handle, nan_grid.data_ptr(), maps.data_ptr(), E, C,
+                  out_img.data_ptr(), s)
+        y = torch.empty((M, K), dtype=torch.complex64, device=dev)
+        _lib.call("tr_data_simulate", plan.handle, 0, img.data_ptr(), maps.data_ptr(), E, C, y.data_ptr(), s)
+        ys.append(y.cpu().numpy())
+    plan.close()
+    assert np.all(np.isfinite(ys[0]))
+    assert np.array_equal(ys[0], ys[1])
