@@ -184,6 +184,17 @@ int tr_tvme_primal(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, c
                    const tr_c64* p_m, const tr_c64* p_ex, const tr_c64* p_ey, const tr_c64* p_ez,
                    float tau, double* partial, void* stream);
 
+/* No counterpart in the reference: tr_tvme_primal followed by tr_reduce_partials, allocating
+ * nothing, for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops or harnesses that forbid hipMalloc).  The per-block partial sums live in a 4 MiB device
+ * array owned by the library (one per GPU, held until the process ends), so B*Z*Y*X may be at most
+ * 2^26 (else -1, nothing launched).  sums: device double[2] receiving (sum |u - u_old|^2,
+ * sum |u|^2), bit-identical to the two-call path.  `stream` must belong to the current device.
+ * Calls that overlap on one GPU share that array and race: serialise them. */
+int tr_tvme_primal_noalloc(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, const tr_c64* grad,
+                           const tr_c64* p_m, const tr_c64* p_ex, const tr_c64* p_ey, const tr_c64* p_ez,
+                           float tau, double* sums, void* stream);
+
 /* TVMW motion dual (lambda1) + db1 echo-wavelet dual (lambda2) (tvmw.py:468-520). */
 int tr_tvmw_dual(int B, int E, int Z, int Y, int X, const tr_c64* ubar, tr_c64* p_m, tr_c64* p_w1,
                  float sigma, float lam1, float lam2, int l2_coupling, void* stream);

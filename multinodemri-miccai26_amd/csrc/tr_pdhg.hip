@@ -6,6 +6,9 @@
 // process (all MPI peers PROC_NULL, ghost slabs zero).  Stopping-rule sums are
 // accumulated in float64 per block and reduced in a fixed order.
 
+#include <mutex>
+#include <string>
+
 #include "toporecon_hip.h"
 #include "tr_common.hpp"
 
@@ -409,6 +412,32 @@ static unsigned ew_blocks(const VolGeom& g) {
   return (unsigned)(((int64_t)g.B * g.N + kEwBlock - 1) / kEwBlock);
 }
 
+// Stopping-sum partials of tr_tvme_primal_noalloc: one float64 pair per 256
+// (bin, voxel) columns, so 2^18 pairs (4 MiB per GPU, held until the process
+// ends) cover B * Z * Y * X up to 64 Mi.  Every partial the reduction reads is
+// written by the primal kernel of the same call, so nothing needs clearing.
+constexpr int64_t kScratchPairs = (int64_t)1 << 18;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double2 scratch[kScratchPairs];
+
+// Device address of `scratch` on the current device, looked up once per device
+// (the symbol's host-side name is not a device address).
+static int scratch_address(double2** out) {
+  static std::mutex mu;
+  static double2* addr[kMaxDevices] = {};
+  int dev = 0;
+  TR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) TR_FAIL(kUnsupported, "device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    TR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = (double2*)p;
+  }
+  *out = addr[dev];
+  return kOk;
+}
+
 }  // namespace tr
 
 using namespace tr;
@@ -521,6 +550,36 @@ int tr_cscale(const tr_c64* x, int64_t n, double s, tr_c64* out, void* stream) {
   if (n == 0) return kOk;
   hipLaunchKernelGGL(k_cscale, dim3((unsigned)((n + kEwBlock - 1) / kEwBlock)), dim3(kEwBlock), 0,
                      (hipStream_t)stream, (const float2*)x, n, (float)s, (float2*)out);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_tvme_primal_noalloc(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64* ubar, const tr_c64* grad,
+                           const tr_c64* p_m, const tr_c64* p_ex, const tr_c64* p_ey, const tr_c64* p_ez,
+                           float tau, double* sums, void* stream) {
+  VolGeom g;
+  if (int r = check_geom(B, E, Z, Y, X, &g)) return r;
+  const int64_t n = ((int64_t)g.B * g.N + kEwBlock - 1) / kEwBlock;
+  if (n > kScratchPairs)
+    TR_FAIL(kInvalidArgument, "B*Z*Y*X = " + std::to_string((int64_t)g.B * g.N) +
+                                  " exceeds the no-allocation limit of " +
+                                  std::to_string(kScratchPairs * kEwBlock) + " (bin, voxel) columns");
+  if (!sums) TR_FAIL(kInvalidArgument, "null sums");
+  hipStream_t s = (hipStream_t)stream;
+  int sdev = 0, cur = 0;
+  TR_HIP(hipGetDevice(&cur));
+  if (s) TR_HIP(hipStreamGetDevice(s, &sdev));
+  if (s && sdev != cur) TR_FAIL(kInvalidArgument, "the stream's device must be the current device");
+  double2* part = nullptr;
+  if (int r = scratch_address(&part)) return r;
+  {
+    ProfScope ps(kKTvmePrimal, 8.0 * 8.0 * g.B * g.E * g.N, s);
+    hipLaunchKernelGGL(k_tvme_primal, dim3(ew_blocks(g)), dim3(kEwBlock), 0, s, g, F2(u), F2(ubar), F2(grad),
+                       F2(p_m), F2(p_ex), F2(p_ey), F2(p_ez), tau, part);
+    TR_LAUNCH_CHECK();
+  }
+  ProfScope ps(kKReduce, 16.0 * n + 16.0, s);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kEwBlock), 0, s, (const double2*)part, (int)n, sums);
   TR_LAUNCH_CHECK();
   return kOk;
 }

@@ -61,6 +61,7 @@ SIGNATURES = {
     "tr_tvm_primal": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp, c_vp, c_f32, c_vp, c_vp]),
     "tr_tvme_dual": (c_int, [c_int] * 5 + [c_vp] * 5 + [c_f32, c_f32, c_f32, c_int, c_vp]),
     "tr_tvme_primal": (c_int, [c_int] * 5 + [c_vp] * 7 + [c_f32, c_vp, c_vp]),
+    "tr_tvme_primal_noalloc": (c_int, [c_int] * 5 + [c_vp] * 7 + [c_f32, c_vp, c_vp]),
     "tr_tvmw_dual": (c_int, [c_int] * 5 + [c_vp] * 3 + [c_f32, c_f32, c_f32, c_int, c_vp]),
     "tr_tvmw_primal": (c_int, [c_int] * 5 + [c_vp] * 6 + [c_f32, c_vp, c_vp]),
     "tr_w2_coeff_count": (c_i64, [c_int, c_int, c_int, c_int]),
