@@ -471,39 +471,66 @@ __device__ __forceinline__ void cl_store(const SpreadArgs& a, int64_t cl, int k,
 }
 
 // Clusters with at most kSpreadSeg pairs (empty clusters are written as 0).
+// With a.seg_blocks > 0 the first seg_blocks workgroups run the heavy-cluster segments (the
+// longest waves first, their sums to a.partial), the rest the light clusters; both share one
+// pair walk so the kernel keeps the register footprint of either.
 template <int KCI, int SORTED>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs a) {
   constexpr int NP = ClLanes<KCI>::NP;
+  constexpr int NPL = ClLanes<KCI>::NPL;
   const int lane = threadIdx.x & 63;
+  const unsigned wave = (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   unsigned blk = blockIdx.x;
-  if (blk < a.xcd_full) {  // runs of xcd_chunk consecutive logical blocks per XCD
-    const unsigned r = blk / kNumXcd, x = blk % kNumXcd, q = a.div_xcd.div(r);
-    blk = q * (kNumXcd * (unsigned)a.xcd_chunk) + x * (unsigned)a.xcd_chunk + (r - q * (unsigned)a.xcd_chunk);
+  const bool seg = blk < a.seg_blocks;
+  int64_t cl, sg = 0;
+  int grp, j0, j1;
+  if (seg) {
+    const int64_t item = (int64_t)blk * 4 + wave;
+    if (item >= a.nseg * a.ngrp) return;
+    const unsigned usg = a.div_ngrp.div((unsigned)item);
+    sg = usg;
+    grp = (int)((unsigned)item - usg * (unsigned)a.ngrp);
+    cl = a.seg_row[sg];
+    j0 = a.seg_j0[sg];
+    j1 = min(j0 + kSpreadSeg, a.row_ptr[cl + 1]);
+  } else {
+    blk -= a.seg_blocks;
+    if (blk < a.xcd_full) {  // runs of xcd_chunk consecutive logical blocks per XCD
+      const unsigned r = blk / kNumXcd, x = blk % kNumXcd, q = a.div_xcd.div(r);
+      blk = q * (kNumXcd * (unsigned)a.xcd_chunk) + x * (unsigned)a.xcd_chunk + (r - q * (unsigned)a.xcd_chunk);
+    }
+    const unsigned item = blk * 4u + wave;
+    const unsigned nitems = a.active ? (unsigned)(a.nactive * a.ngrp) : (unsigned)(a.NCL * a.ngrp);
+    if (item >= nitems) return;
+    const unsigned uit = a.div_ngrp.div(item);
+    grp = (int)(item - uit * (unsigned)a.ngrp);
+    cl = a.active ? (unsigned)a.active[uit] : uit;
+    j0 = a.row_ptr[cl];
+    j1 = a.row_ptr[cl + 1];
+    if (!a.active && j1 - j0 > kSpreadSeg) return;  // heavy cluster: written by k_spread_combine
   }
-  const unsigned item = blk * 4u + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (item >= (unsigned)(a.NCL * a.ngrp)) return;
-  const unsigned ucl = a.div_ngrp.div(item);
-  const int64_t cl = ucl;
-  const int grp = (int)(item - ucl * (unsigned)a.ngrp);
-  const int j0 = a.row_ptr[cl], j1 = a.row_ptr[cl + 1];
-  if (j1 - j0 > kSpreadSeg) return;  // heavy cluster: written by k_spread_combine
   const int k = grp * KCI + lane % KCI;
   const int pl = lane / KCI;
   float2 acc[NP];
 #pragma unroll
   for (int i = 0; i < NP; ++i) acc[i] = make_float2(0.f, 0.f);
   cl_run<KCI, SORTED>(a, j0, j1, k, pl, acc);
-  cl_store<KCI>(a, cl, k, pl, acc);
+  if (!seg) {
+    cl_store<KCI>(a, cl, k, pl, acc);
+  } else if (k < a.K) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const int pt = pl + i * NPL;
+      if (pt < kClPts) a.partial[(sg * kClPts + pt) * a.K + k] = acc[i];
+    }
+  }
 }
 
 // One segment of a heavy cluster -> partial[seg][point][K].
 template <int KCI, int SORTED>
-__global__ void __launch_bounds__(256) k_spread_segments(SpreadArgs a) {
+__device__ __forceinline__ void spread_segment(const SpreadArgs& a, int64_t item, int lane) {
   constexpr int NP = ClLanes<KCI>::NP;
   constexpr int NPL = ClLanes<KCI>::NPL;
-  const int lane = threadIdx.x & 63;
-  const int64_t item = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (item >= a.nseg * a.ngrp) return;
   const int64_t sg = item / a.ngrp;
   const int grp = (int)(item - sg * a.ngrp);
   const int cl = a.seg_row[sg];
@@ -523,19 +550,28 @@ __global__ void __launch_bounds__(256) k_spread_segments(SpreadArgs a) {
   }
 }
 
-// Heavy cluster = sum of its segment partials in segment order; one thread
-// per (heavy cluster, point, channel), channel fastest.
+template <int KCI, int SORTED>
+__global__ void __launch_bounds__(256) k_spread_segments(SpreadArgs a) {
+  const int64_t item = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.nseg * a.ngrp) return;
+  spread_segment<KCI, SORTED>(a, item, threadIdx.x & 63);
+}
+
+// Heavy cluster (or cell) = sum of its segment partials in segment order; one
+// thread per (heavy cluster, point, channel), channel fastest.
+template <int CZ, int CY, int CX>
 __global__ void __launch_bounds__(256) k_spread_combine(SpreadArgs a) {
+  constexpr int NPT = CZ * CY * CX;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int K = a.K;
-  if (i >= a.nheavy * kClPts * K) return;
+  if (i >= a.nheavy * NPT * K) return;
   const int k = (int)(i % K);
   const int64_t hp = i / K;
-  const int pt = (int)(hp % kClPts);
-  const int64_t h = hp / kClPts;
+  const int pt = (int)(hp % NPT);
+  const int64_t h = hp / NPT;
   const int s0 = a.heavy_seg_ptr[h], s1 = a.heavy_seg_ptr[h + 1];
   const float2* src = a.partial + (int64_t)pt * K + k;
-  const int64_t stride = (int64_t)kClPts * K;
+  const int64_t stride = (int64_t)NPT * K;
   float2 acc = make_float2(0.f, 0.f);
   int sg = s0;
   for (; sg + 4 <= s1; sg += 4) {
@@ -551,9 +587,9 @@ __global__ void __launch_bounds__(256) k_spread_combine(SpreadArgs a) {
   const int cz = (int)(cl / ((int64_t)ncy * ncx));
   const int rem = (int)(cl - (int64_t)cz * ncy * ncx);
   const int cy = rem / ncx, cx = rem - (rem / ncx) * ncx;
-  const int z = cz * kClZ + pt / (kClY * kClX);
-  const int y = cy * kClY + (pt / kClX) % kClY;
-  const int x = cx * kClX + pt % kClX;
+  const int z = cz * CZ + pt / (CY * CX);
+  const int y = cy * CY + (pt / CX) % CY;
+  const int x = cx * CX + pt % CX;
   const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
   if (z < gz && y < gy && x < gx && (!a.col_win || in_window(z, a.col_win[y * gx + x], gz)))
     a.grid[(((int64_t)z * gy + y) * gx + x) * K + k] = acc;
@@ -699,9 +735,8 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
     if (int r = ensure_partial(p, b.nseg * (int64_t)kClPts * K)) return r;
     a.partial = p->partial;
   }
-#define TR_SPREAD_KCI(KERN, SRC, n)                                                                \
+#define TR_SPREAD_KCI(KERN, SRC, blocks)                                                           \
   do {                                                                                            \
-    const unsigned blocks = grid1d((n) * a.ngrp, 4);                                              \
     switch (kci) {                                                                                \
       case 1: hipLaunchKernelGGL((KERN<1, SRC>), dim3(blocks), dim3(256), 0, s, a); break;        \
       case 2: hipLaunchKernelGGL((KERN<2, SRC>), dim3(blocks), dim3(256), 0, s, a); break;        \
@@ -713,20 +748,29 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
     }                                                                                             \
     TR_LAUNCH_CHECK();                                                                            \
   } while (0)
-#define TR_SPREAD_ALL(KERN, n)                          \
-  do {                                                  \
-    if (a.sorted_src) TR_SPREAD_KCI(KERN, 1, n);        \
-    else TR_SPREAD_KCI(KERN, 0, n);                     \
+#define TR_SPREAD_ALL(KERN, blocks)                          \
+  do {                                                       \
+    if (a.sorted_src) TR_SPREAD_KCI(KERN, 1, blocks);        \
+    else TR_SPREAD_KCI(KERN, 0, blocks);                     \
   } while (0)
+  // With support pruning and a visiting order only the ordered active clusters run, and the
+  // heavy-cluster segments (the longest waves) lead them in the same launch.
+  const bool ordered = a.col_win && b.corder;
+  a.active = ordered ? b.corder : nullptr;
+  a.nactive = ordered ? b.ncorder : 0;
+  const int64_t nlight = ordered ? b.ncorder : a.NCL;
+  const unsigned light_blocks = grid1d(nlight * a.ngrp, 4);
   {
-    const unsigned S = (unsigned)a.xcd_chunk, nb = grid1d(a.NCL * a.ngrp, 4);
+    const unsigned S = (unsigned)a.xcd_chunk;
     a.div_xcd = make_fastdiv(S ? S : 1u);
-    a.xcd_full = S ? nb / (kNumXcd * S) * (kNumXcd * S) : 0u;
+    a.xcd_full = S ? light_blocks / (kNumXcd * S) * (kNumXcd * S) : 0u;
   }
-  TR_SPREAD_ALL(k_spread, a.NCL);
+  a.seg_blocks = ordered ? grid1d(b.nseg * a.ngrp, 4) : 0u;
+  if (a.seg_blocks + light_blocks > 0) TR_SPREAD_ALL(k_spread, a.seg_blocks + light_blocks);
   if (b.nseg > 0) {
-    TR_SPREAD_ALL(k_spread_segments, b.nseg);
-    hipLaunchKernelGGL(k_spread_combine, dim3(grid1d(b.nheavy * kClPts * K, 256)), dim3(256), 0, s, a);
+    if (!ordered) TR_SPREAD_ALL(k_spread_segments, grid1d(b.nseg * a.ngrp, 4));
+    hipLaunchKernelGGL((k_spread_combine<kClZ, kClY, kClX>), dim3(grid1d(b.nheavy * kClPts * K, 256)), dim3(256),
+                       0, s, a);
     TR_LAUNCH_CHECK();
   }
 #undef TR_SPREAD_ALL

@@ -23,7 +23,11 @@ namespace tr {
 constexpr int kWidth = 4;
 constexpr int kTile = 8;  // sample sort tile edge (grid points)
 constexpr int kMaxKc = 16;  // channels per FFT workgroup
-constexpr int kSpreadSeg = 256;  // samples per heavy-cluster segment
+#ifndef TR_SPREAD_SEG
+#define TR_SPREAD_SEG 1024
+#endif
+constexpr int kSpreadSeg = TR_SPREAD_SEG;  // pairs per heavy-cluster segment
+constexpr int kSpreadOrderLongestFirst = 0xffff;  // TR_PARAM_SPREAD_ORDER code of the default order
 #ifndef TR_SPREAD_UNROLL
 #define TR_SPREAD_UNROLL 4
 #endif
@@ -77,6 +81,10 @@ struct BinData {
   int64_t ngroups = 0;
   int* grp = nullptr;            // ngroups + 1 offsets into the sorted samples
   TapRec* tapg = nullptr;
+  // light clusters holding at least one support point, in the visiting order of the cluster
+  // kernel (TR_PARAM_SPREAD_ORDER); empty = every cluster in index order
+  int64_t ncorder = 0;
+  int* corder = nullptr;
 };
 
 }  // namespace tr
@@ -121,6 +129,7 @@ struct tr_plan {
   int spread_xcd_chunk = 0;
   int lines_xcd_lines = 4;  // y / z passes: whole lines per XCD run (0 = round-robin)
   int interp_kgroups = 0;  // 1: one 64-channel group per launch row (group-major order)
+  int spread_order = 0;    // 0: all clusters in index order; else the code of the active-cluster order
   // k-space support of all bins' taps: per (y, x) column of the oversampled
   // grid (natural order) the window (start, len) of z positions any tap
   // touches; len 0 = empty column.  Grid points outside it are never read by
@@ -289,6 +298,10 @@ struct SpreadArgs {
   FastDiv div_ngrp, div_plane, div_ncx, div_xcd;
   unsigned xcd_full;  // blocks below this are remapped in XCD runs of xcd_chunk
   const int2* col_win;  // support windows (null = write every point)
+  // visiting order of the light clusters (null: every cluster in index order)
+  const int* active;
+  int64_t nactive;
+  unsigned seg_blocks;  // leading workgroups of the cluster kernel that run the heavy segments
 };
 
 // Position p of an axis of length g lies in the window (start, len) of that axis.
@@ -353,6 +366,7 @@ int ensure_partial(tr_plan* p, int64_t n);
 int ensure_work(tr_plan* p, int64_t K);
 int build_support(tr_plan* p, hipStream_t s);
 int build_tap_groups(tr_plan* p, BinData& b, hipStream_t s);
+int build_cluster_order(tr_plan* p, BinData& b, int code);
 void use_slot(tr_plan* p, int slot);
 int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t s);
 int count_touched(const tr_plan* p, BinData& b, hipStream_t s);

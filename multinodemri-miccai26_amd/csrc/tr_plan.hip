@@ -580,6 +580,61 @@ int build_support(tr_plan* p, hipStream_t s) {
   return kOk;
 }
 
+// Visiting order of the cluster kernel: the light clusters that hold at least one support
+// point (heavy ones are written by the segment combine, clusters without support points by
+// nobody), grouped in bricks of bz x by x bx clusters (code = bz | by << 4 | bx << 8; 1 x 1 x 1 =
+// index order), bricks and the clusters inside a brick in index order.  A sample's taps touch
+// 2 x 2 x 3 neighbouring clusters, so a brick keeps most of them in one XCD's run of blocks.
+int build_cluster_order(tr_plan* p, BinData& b, int code) {
+  (void)hipFree(b.corder);
+  b.corder = nullptr;
+  b.ncorder = 0;
+  const bool lpt = code == kSpreadOrderLongestFirst;
+  const int bz = lpt ? 1 : std::max(1, code & 15), by = lpt ? 1 : std::max(1, (code >> 4) & 15),
+            bx = lpt ? 1 : std::max(1, (code >> 8) & 255);
+  std::vector<int> rp(p->NCL + 1);
+  TR_HIP(hipMemcpy(rp.data(), b.row_ptr, sizeof(int) * (p->NCL + 1), hipMemcpyDeviceToHost));
+  const int ncz = p->ncl[0], ncy = p->ncl[1], ncx = p->ncl[2];
+  const int gz = p->g[0], gy = p->g[1], gx = p->g[2];
+  const std::vector<int2>& win = p->col_win_host;
+  std::vector<int> order;
+  order.reserve((size_t)p->NCL);
+  for (int z0 = 0; z0 < ncz; z0 += bz)
+    for (int y0 = 0; y0 < ncy; y0 += by)
+      for (int x0 = 0; x0 < ncx; x0 += bx)
+        for (int cz = z0; cz < std::min(z0 + bz, ncz); ++cz)
+          for (int cy = y0; cy < std::min(y0 + by, ncy); ++cy)
+            for (int cx = x0; cx < std::min(x0 + bx, ncx); ++cx) {
+              const int64_t t = ((int64_t)cz * ncy + cy) * ncx + cx;
+              const int len = rp[t + 1] - rp[t];
+              if (len > kSpreadSeg) continue;
+              bool live = len > 0 || win.empty();
+              for (int iy = 0; iy < kClY && !live; ++iy)
+                for (int ix = 0; ix < kClX && !live; ++ix) {
+                  const int y = cy * kClY + iy, x = cx * kClX + ix;
+                  if (y >= gy || x >= gx) continue;
+                  const int2 w = win[(size_t)y * gx + x];
+                  for (int iz = 0; iz < kClZ && !live; ++iz) {
+                    const int zz = cz * kClZ + iz;
+                    if (zz >= gz || w.y <= 0) continue;
+                    int d = zz - w.x;
+                    if (d < 0) d += gz;
+                    live = d < w.y;
+                  }
+                }
+              if (live) order.push_back((int)t);
+            }
+  if (code == kSpreadOrderLongestFirst)  // longest pair lists first (index order among equal lengths)
+    std::stable_sort(order.begin(), order.end(),
+                     [&rp](int x, int y) { return rp[x + 1] - rp[x] > rp[y + 1] - rp[y]; });
+  b.ncorder = (int64_t)order.size();
+  if (!order.empty()) {
+    TR_HIP(hipMalloc(&b.corder, sizeof(int) * order.size()));
+    TR_HIP(hipMemcpy(b.corder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+  }
+  return kOk;
+}
+
 void free_plan(tr_plan* p) {
   if (!p) return;
   for (auto& b : p->bins) {
@@ -588,6 +643,7 @@ void free_plan(tr_plan* p) {
     (void)hipFree(b.tap);
     (void)hipFree(b.grp);
     (void)hipFree(b.tapg);
+    (void)hipFree(b.corder);
     (void)hipFree(b.row_ptr);
     (void)hipFree(b.pairs);
     (void)hipFree(b.seg_row);
@@ -692,6 +748,9 @@ int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float
   }
   int r = build_support(p, s);
   for (int b = 0; b < nbins && !r; ++b) r = build_tap_groups(p, p->bins[b], s);
+  for (int b = 0; b < nbins && !r; ++b)
+    if (p->bins[b].M > 0) r = build_cluster_order(p, p->bins[b], kSpreadOrderLongestFirst);
+  if (!r) p->spread_order = kSpreadOrderLongestFirst;
   if (r) {
     std::string msg = last_error();
     free_plan(p);
@@ -768,6 +827,21 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
       if (value < 0 || value > 64) TR_FAIL(kInvalidArgument, "lines per XCD run outside [0, 64]");
       p->lines_xcd_lines = (int)value;
       return kOk;
+    case TR_PARAM_SPREAD_ORDER: {
+      if (value < 0 || value > 0xffff) TR_FAIL(kInvalidArgument, "spreading order code outside [0, 65535]");
+      TR_HIP(hipDeviceSynchronize());
+      for (BinData& b : p->bins) {
+        if (value == 0 || b.M == 0) {
+          (void)hipFree(b.corder);
+          b.corder = nullptr;
+          b.ncorder = 0;
+        } else if (int r = build_cluster_order(p, b, (int)value)) {
+          return r;
+        }
+      }
+      p->spread_order = (int)value;
+      return kOk;
+    }
     case TR_PARAM_SUPPORT_POINTS:
     case TR_PARAM_LIVE_COLUMNS:
       TR_FAIL(kInvalidArgument, "read-only plan parameter " + std::to_string(param));
@@ -786,6 +860,7 @@ int tr_plan_get_param(const tr_plan* p, int param, int64_t* value) {
     case TR_PARAM_LINES_XCD_LINES: *value = p->lines_xcd_lines; return kOk;
     case TR_PARAM_SUPPORT_POINTS: *value = p->support_points; return kOk;
     case TR_PARAM_LIVE_COLUMNS: *value = p->live_cols; return kOk;
+    case TR_PARAM_SPREAD_ORDER: *value = p->spread_order; return kOk;
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));
   }
 }

@@ -124,7 +124,8 @@ class NufftPlan:
         _lib.call("tr_plan_release_workspace", self.handle)
 
     PARAMS = {"interp_xcd_chunk": 1, "spread_xcd_chunk": 2, "prune": 3, "support_points": 4,
-              "live_columns": 5, "interp_kgroups": 6, "workspace_slots": 7, "lines_xcd_lines": 8}
+              "live_columns": 5, "interp_kgroups": 6, "workspace_slots": 7, "lines_xcd_lines": 8,
+              "spread_order": 10}
 
     def set_param(self, name: str, value: int):
         """Set a launch parameter (block order only; results are unchanged)."""

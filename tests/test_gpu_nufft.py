@@ -249,3 +249,46 @@ def test_grouped_interp_never_reads_unwritten_workspace(torch_mod):
     plan.close()
     assert np.all(np.isfinite(ys[0]))
     assert np.array_equal(ys[0], ys[1])
+
+
+def test_spreading_orders_and_heavy_segments(torch_mod):
+    """The spreading's cluster visiting orders (longest first, bricks, index order over every
+    cluster) and support pruning give identical bits, also for clusters whose pair lists are long
+    enough to be summed in segments (every cone interleave starts at k = 0, so the centre cluster
+    collects thousands of pairs), and the result matches the oracle."""
+    torch = torch_mod
+    from toporecon_hip import _lib, synth
+    from toporecon_hip.nufft_backend import NufftPlan
+    print("spreading orders / heavy segments", flush=True)
+    shape = (12, 16, 16)
+    coord = synth.cone_trajectory(3000, 24, shape).reshape(-1, 3)
+    dev = torch.device("cuda", 0)
+    plan = NufftPlan(shape, [torch.from_numpy(coord).to(dev)], dev)
+    assert plan.get_param("spread_order") == 0xffff
+    E, C = 2, 3
+    K = E * C
+    M = plan.samples(0)
+    rng = np.random.default_rng(21)
+    maps = torch.from_numpy(_cplx(rng, (C, *shape))).to(dev)
+    q = _cplx(rng, (K, M))
+    dcf = rng.uniform(0.2, 1.0, M).astype(np.float32)
+    qt = torch.from_numpy(q).to(dev)
+    dt = torch.from_numpy(dcf).to(dev)
+    s = _lib.stream_handle(dev)
+    outs = []
+    for order, prune in ((0xffff, 1), (0x111, 1), (0x822, 1), (0, 1), (0xffff, 0)):
+        plan.set_param("spread_order", order)
+        plan.set_param("prune", prune)
+        g = torch.empty((E, *shape), dtype=torch.complex64, device=dev)
+        _lib.call("tr_nufft_adjoint", plan.handle, 0, qt.data_ptr(), dt.data_ptr(), maps.data_ptr(), E, C,
+                  g.data_ptr(), 0, s)
+        outs.append(g.cpu().numpy())
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
+    mp = maps.cpu().numpy().astype(np.complex128)
+    for e in range(E):
+        ref = sum(np.conj(mp[c]) * on.nufft_adjoint(q[e * C + c] * dcf, coord, shape) for c in range(C))
+        assert _rel(outs[0][e], ref) < TOL
+    with pytest.raises(Exception):
+        plan.set_param("spread_order", 70000)
+    plan.close()
