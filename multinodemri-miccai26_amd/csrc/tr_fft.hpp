@@ -17,14 +17,32 @@
 
 namespace tr {
 
-constexpr int kBlock = 256;      // threads per FFT workgroup
-constexpr int kMaxFftLen = 256;  // longest supported axis (oversampled grid)
-constexpr int kMaxStages = 12;
+constexpr int kBlock = 256;       // threads per FFT workgroup
+constexpr int kPosPerWorker = 16;  // line positions per worker: a chunk of KC channels has
+                                   // kBlock / KC workers, so lines of up to 16 kBlock / KC points
+constexpr int kMaxLine = kPosPerWorker * kBlock;  // longest line a one-channel workgroup holds
+constexpr int kMaxTransform = 2048;  // longest transform accepted (its tile and a row of the x pass
+                                     // fit the 64 KiB of LDS a workgroup may use)
+constexpr int kMaxStages = 14;
 
+// Longest transform a workgroup of KC channels holds.
+__host__ __device__ constexpr int max_line(int kc) { return kMaxLine / kc; }
+
+// One axis of the oversampled grid.  Lengths whose prime factors are at most 13
+// are transformed directly (M = L).  Others use Bluestein's chirp-z identity
+// nk = (n^2 + k^2 - (k - n)^2) / 2: X_k = w_k sum_n (x_n w_n) conj(w_{k-n}) with
+// w_m = exp(-i pi m^2 / L), a circular convolution evaluated with two forward
+// FFTs of a length M >= 2L - 1 whose factors are 2, 3 and 5; chirp[n] = w_n
+// (n < L) and hhat = FFT_M(b) / M with b_m = conj(w_m), b_{M-m} = conj(w_m)
+// (0 <= m < L), both computed in float64 on the host.
 struct FftAxis {
-  int L;
-  int nstages;
+  int L;        // logical length
+  int M;        // transform length (L, or the Bluestein convolution length)
+  int bluestein;
+  int nstages;  // radices of the length-M transform
   int radix[kMaxStages];
+  const float2* chirp;  // device, L values (Bluestein only)
+  const float2* hhat;   // device, M values (Bluestein only)
 };
 
 __host__ __device__ constexpr int lds_ld(int kc) { return kc >= 4 ? kc + 1 : kc; }
@@ -136,7 +154,7 @@ struct Dft {
 template <int R, int W>
 __device__ __forceinline__ void fft_stage(float2* buf, int ld, int ch, int w, int L, int Ns,
                                           const float2* tw) {
-  constexpr int NB = (kMaxFftLen / R + W - 1) / W;
+  constexpr int NB = (kPosPerWorker * W / R + W - 1) / W;
   const int nb = L / R;
   const int tstride = L / (Ns * R);
   float2 v[NB][R];
@@ -168,27 +186,56 @@ __device__ __forceinline__ void fft_stage(float2* buf, int ld, int ch, int w, in
   __syncthreads();
 }
 
-// Full forward FFT of the line in buf (natural order in, natural order out).
-// Must be called by every thread of the block (contains barriers).
+// Forward FFT of length ax.M of the line in buf (natural order in and out).
 template <int W>
-__device__ __forceinline__ void fft_line(float2* buf, int ld, int ch, int w, const FftAxis& ax,
-                                         const float2* tw) {
+__device__ __forceinline__ void fft_stages(float2* buf, int ld, int ch, int w, const FftAxis& ax,
+                                           const float2* tw) {
   int Ns = 1;
   for (int s = 0; s < ax.nstages; ++s) {
     const int R = ax.radix[s];
     switch (R) {
-      case 2: fft_stage<2, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
-      case 3: fft_stage<3, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
-      case 4: fft_stage<4, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
-      case 5: fft_stage<5, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
-      case 7: fft_stage<7, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
-      case 8: fft_stage<8, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
-      case 11: fft_stage<11, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
-      case 13: fft_stage<13, W>(buf, ld, ch, w, ax.L, Ns, tw); break;
+      case 2: fft_stage<2, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
+      case 3: fft_stage<3, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
+      case 4: fft_stage<4, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
+      case 5: fft_stage<5, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
+      case 7: fft_stage<7, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
+      case 8: fft_stage<8, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
+      case 11: fft_stage<11, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
+      case 13: fft_stage<13, W>(buf, ld, ch, w, ax.M, Ns, tw); break;
       default: break;  // rejected on the host
     }
     Ns *= R;
   }
+}
+
+// Full forward DFT of length ax.L of the line in buf (positions 0 .. L-1 in and
+// out; a Bluestein axis uses positions L .. M-1 as scratch).  Must be called by
+// every thread of the block (contains barriers).
+template <int W>
+__device__ __forceinline__ void fft_line(float2* buf, int ld, int ch, int w, const FftAxis& ax,
+                                         const float2* tw) {
+  if (!ax.bluestein) {
+    fft_stages<W>(buf, ld, ch, w, ax, tw);
+    return;
+  }
+  const int L = ax.L, M = ax.M;
+  for (int n = w; n < M; n += W) {  // a = x w, zero-padded to M
+    float2& x = buf[n * ld + ch];
+    x = n < L ? cmul(x, ax.chirp[n]) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_stages<W>(buf, ld, ch, w, ax, tw);
+  for (int k = w; k < M; k += W) {  // conj(FFT(a) FFT(b) / M)
+    float2& x = buf[k * ld + ch];
+    x = cconj(cmul(x, ax.hhat[k]));
+  }
+  __syncthreads();
+  fft_stages<W>(buf, ld, ch, w, ax, tw);  // conj of the circular convolution a (*) b
+  for (int k = w; k < L; k += W) {
+    float2& x = buf[k * ld + ch];
+    x = cmul(cconj(x), ax.chirp[k]);
+  }
+  __syncthreads();
 }
 
 // ---- compile-time plans ------------------------------------------------------
@@ -222,7 +269,7 @@ constexpr bool ct_supported(int L) {
   int n = L;
   for (int i = 0; i < 8; ++i)
     while (n % kRadixOrder[i] == 0) n /= kRadixOrder[i];
-  return n == 1 && L <= kMaxFftLen;
+  return n == 1 && L <= max_line(16);
 }
 
 template <int R, int W, int L, int Ns>
@@ -279,15 +326,13 @@ __device__ __forceinline__ void fft_any(float2* buf, int ld, int ch, int w, cons
   }
 }
 
-// Host: factor L into supported radices (8s first, then 4, 2, odd primes).
-inline bool make_fft_axis(int L, FftAxis* ax) {
-  if (L < 1 || L > kMaxFftLen) return false;
-  ax->L = L;
-  ax->nstages = 0;
-  int n = L;
+// Host: factor n into supported radices (8s first, then 4, 2, odd primes);
+// false if a prime factor above 13 remains.
+inline bool factor_radices(int n, int* nstages, int* radix) {
+  *nstages = 0;
   auto take = [&](int r) {
-    while (n % r == 0 && ax->nstages < kMaxStages) {
-      ax->radix[ax->nstages++] = r;
+    while (n % r == 0 && *nstages < kMaxStages) {
+      radix[(*nstages)++] = r;
       n /= r;
     }
   };
@@ -300,6 +345,32 @@ inline bool make_fft_axis(int L, FftAxis* ax) {
   take(11);
   take(13);
   return n == 1;
+}
+
+// Host: the transform of an axis of length L (tables are filled in by the plan).
+// Direct when L's prime factors are at most 13 and L <= kMaxTransform; otherwise
+// Bluestein with the smallest M = 2^a 3^b 5^c >= 2L - 1 (at most kMaxTransform).
+inline bool make_fft_axis(int L, FftAxis* ax) {
+  if (L < 1) return false;
+  ax->L = L;
+  ax->chirp = nullptr;
+  ax->hhat = nullptr;
+  if (L <= kMaxTransform && factor_radices(L, &ax->nstages, ax->radix)) {
+    ax->M = L;
+    ax->bluestein = 0;
+    return true;
+  }
+  int M = 0;
+  for (int m = 2 * L - 1; m <= kMaxTransform && !M; ++m) {
+    int r = m;
+    for (int f : {2, 3, 5})
+      while (r % f == 0) r /= f;
+    if (r == 1) M = m;
+  }
+  if (!M) return false;
+  ax->M = M;
+  ax->bluestein = 1;
+  return factor_radices(M, &ax->nstages, ax->radix);
 }
 
 }  // namespace tr

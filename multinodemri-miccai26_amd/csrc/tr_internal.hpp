@@ -109,6 +109,8 @@ struct tr_plan {
   float pad_scale;  // 1/sqrt(N)
   tr::FftAxis ax[3];
   float2* tw[3] = {nullptr, nullptr, nullptr};
+  float2* chirp[3] = {nullptr, nullptr, nullptr};  // Bluestein axes only
+  float2* hhat[3] = {nullptr, nullptr, nullptr};
   float* apod[3] = {nullptr, nullptr, nullptr};
   std::vector<tr::BinData> bins;
   float2* work = nullptr;  // grid workspace, G * work_k complex
@@ -164,7 +166,8 @@ struct FwdXArgs {
   const float* apod_x;
   float scale;
   int C;
-  int K;  // grid channel count (E * C)
+  int K;  // grid channel count (stride of a grid point's channel record)
+  int E;  // echoes of this launch: img holds E images, grid channels e * C + c
   float2* grid;
 };
 
@@ -234,10 +237,13 @@ struct AdjXArgs {
   const float* apod_x;
   float scale;
   int C;
-  int K;
-  float2* out;  // (E, Z, Y, X)
+  int K;  // channel stride of the grid
+  int E;  // echoes of this launch (at most kAdjXEchoes): grid channels e * C + c, out (E, Z, Y, X)
+  float2* out;
   int accumulate;
 };
+
+constexpr int kAdjXEchoes = 8;  // echo accumulators of the inverse x pass
 
 enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2, kInterpResid = 3 };
 
@@ -318,7 +324,7 @@ template <int KC, int LC>
 struct Tile {
   static constexpr int ld = lds_ld(KC);
   static constexpr int W = kBlock / KC;  // workers per channel
-  static constexpr int PER = ((LC > 0 ? LC : kMaxFftLen) + W - 1) / W;  // positions per thread
+  static constexpr int PER = ((LC > 0 ? LC : max_line(KC)) + W - 1) / W;  // positions per thread
 };
 
 // ---------------------------------------------------------------------------
@@ -347,9 +353,15 @@ inline int pow2ceil(int v) {
 
 // Channel chunk of the FFT passes: at most 16 channels per workgroup keeps the
 // Stockham stages under ~130 VGPRs and the LDS tile at L * 17 * 8 bytes.
-inline int fft_kc(int K) { return K >= kMaxKc ? kMaxKc : pow2ceil(K); }
+// ... and at most max_line(kc) >= M points per line, so a chunk's tile stays within
+// kMaxLine * 8 bytes (about 34 KiB with the twiddles) whatever the length.
+inline int fft_kc(int K, int M) {
+  int kc = K >= kMaxKc ? kMaxKc : pow2ceil(K);
+  while (kc > 1 && M > max_line(kc)) kc >>= 1;
+  return kc;
+}
 
-inline size_t fft_smem(int L, int kc) { return sizeof(float2) * (size_t)L * (1 + lds_ld(kc)); }
+inline size_t fft_smem(int M, int kc) { return sizeof(float2) * (size_t)M * (1 + lds_ld(kc)); }
 
 
 inline GridGeom geom(const tr_plan* p) {

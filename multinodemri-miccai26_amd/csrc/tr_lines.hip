@@ -94,9 +94,10 @@ template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
-  const int L = LC > 0 ? LC : a.ax.L;
+  const int L = LC > 0 ? LC : a.ax.L;   // line length
+  const int LF = LC > 0 ? LC : a.ax.M;  // transform length (tile rows, twiddles)
   float2* tw = smem;
-  float2* buf = smem + L;
+  float2* buf = smem + LF;
   const int tid = threadIdx.x;
   const int ch = tid % KC, w = tid / KC;
   // block -> (line, channel chunk); runs of xcd_chunk consecutive logical
@@ -137,7 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
   auto live_pos = [&](int p) { return !a.line_win || in_window(p, lw, L); };
   const bool live = k0 + ch < a.K;
   float2* base = a.data + ob * a.outer_stride + in * a.inner_stride + k0 + ch;
-  for (int i = tid; i < L; i += kBlock) tw[i] = a.tw[i];
+  for (int i = tid; i < LF; i += kBlock) tw[i] = a.tw[i];
   if constexpr (two_level_ok<KC, LC>()) {
     if (a.off32) {
       lines_two_level<KC, LC>(a, tw, buf, ch, w, live, lw,

@@ -4,6 +4,8 @@
 // deterministic CSR gather for the adjoint spreading.  Host orchestration and
 // the NUFFT / data-term C entry points.
 
+#include <algorithm>
+
 #include "tr_internal.hpp"
 
 namespace tr {
@@ -32,18 +34,24 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     a.scale = p->pad_scale;
     a.C = C;
     a.K = K;
-    a.grid = p->work;
-    const int kc = fft_kc(C);
+    const int kc = fft_kc(C, p->ax[2].M);
     dim3 grid((unsigned)(nz * ny * ((C + kc - 1) / kc)));
     ProfScope ps(kKFwdXPad, 8.0 * (E + C) * p->N + 8.0 * K * nz * ny * gx, s);
-    // LDS: twiddles + FFT tile + the E image rows of the block (staged once for its coils)
-    const size_t smem = fft_smem(gx, kc) + sizeof(float2) * (size_t)E * p->n[2];
-    if (smem > 65536) TR_FAIL(kUnsupported, "too many echoes per forward launch (LDS)");
-    if (int r = launch_fwd_x(kc, gx, grid, smem, s, a)) return r;
+    // LDS: twiddles + FFT tile + the image rows of the launch's echoes (staged once for the
+    // block's coils); echoes go in groups that fit
+    const size_t tile = fft_smem(p->ax[2].M, kc), row = sizeof(float2) * (size_t)p->n[2];
+    if (tile + row > 65536) TR_FAIL(kUnsupported, "image rows too long for the forward x pass (LDS)");
+    const int eg = (int)std::min<size_t>((size_t)E, (65536 - tile) / row);
+    for (int e0 = 0; e0 < E; e0 += eg) {
+      a.E = std::min(eg, E - e0);
+      a.img = img + (int64_t)e0 * p->N;
+      a.grid = p->work + (int64_t)e0 * C;
+      if (int r = launch_fwd_x(kc, gx, grid, tile + row * a.E, s, a)) return r;
+    }
   }
-  const int kc = fft_kc(K);
-  const int nchunk = (K + kc - 1) / kc;
   {  // y pass over the nz occupied z-planes, windowed load along y
+    const int kc = fft_kc(K, p->ax[1].M);
+    const int nchunk = (K + kc - 1) / kc;
     LinesArgs a;
     a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[1];
@@ -66,9 +74,11 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     dim3 grid((unsigned)(nz * gx * nchunk));
     ProfScope ps(kKFftYFwd, 8.0 * K * nz * ((double)gx * ny + cols), s);
-    if (int r = launch_lines(kc, gy, grid, fft_smem(gy, kc), s, a)) return r;
+    if (int r = launch_lines(kc, gy, grid, fft_smem(p->ax[1].M, kc), s, a)) return r;
   }
   {  // z pass over every (y, x) line, windowed load along z
+    const int kc = fft_kc(K, p->ax[0].M);
+    const int nchunk = (K + kc - 1) / kc;
     LinesArgs a;
     a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[0];
@@ -92,7 +102,7 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     const double pts = prune ? (double)p->support_points : (double)p->G;
     dim3 grid((unsigned)(gy * gx * nchunk));
     ProfScope ps(kKFftZFwd, 8.0 * K * (cols * nz + pts), s);
-    if (int r = launch_lines(kc, gz, grid, fft_smem(gz, kc), s, a)) return r;
+    if (int r = launch_lines(kc, gz, grid, fft_smem(p->ax[0].M, kc), s, a)) return r;
   }
   return kOk;
 }
@@ -104,9 +114,9 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
   const int K = E * C;
   const int nz = p->n[0], ny = p->n[1];
   const int gz = p->g[0], gy = p->g[1], gx = p->g[2];
-  const int kc = fft_kc(K);
-  const int nchunk = (K + kc - 1) / kc;
   {
+    const int kc = fft_kc(K, p->ax[0].M);
+    const int nchunk = (K + kc - 1) / kc;
     LinesArgs a;
     a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[0];
@@ -130,9 +140,11 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     const double pts = prune ? (double)p->support_points : (double)p->G;
     dim3 grid((unsigned)(gy * gx * nchunk));
     ProfScope ps(kKFftZInv, 8.0 * K * (cols * nz + pts), s);
-    if (int r = launch_lines(kc, gz, grid, fft_smem(gz, kc), s, a)) return r;
+    if (int r = launch_lines(kc, gz, grid, fft_smem(p->ax[0].M, kc), s, a)) return r;
   }
   {
+    const int kc = fft_kc(K, p->ax[1].M);
+    const int nchunk = (K + kc - 1) / kc;
     LinesArgs a;
     a.xcd_chunk = (unsigned)(p->lines_xcd_lines * nchunk);
     a.ax = p->ax[1];
@@ -155,14 +167,13 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     const double cols = prune ? (double)p->live_cols : (double)gy * gx;
     dim3 grid((unsigned)(nz * gx * nchunk));
     ProfScope ps(kKFftYInv, 8.0 * K * nz * ((double)gx * ny + cols), s);
-    if (int r = launch_lines(kc, gy, grid, fft_smem(gy, kc), s, a)) return r;
+    if (int r = launch_lines(kc, gy, grid, fft_smem(p->ax[1].M, kc), s, a)) return r;
   }
   {
     AdjXArgs a;
     a.gg = geom(p);
     a.ax = p->ax[2];
     a.tw = p->tw[2];
-    a.grid = p->work;
     a.maps = maps;
     a.apod_z = p->apod[0];
     a.apod_y = p->apod[1];
@@ -170,13 +181,16 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     a.scale = p->pad_scale;
     a.C = C;
     a.K = K;
-    a.out = out;
     a.accumulate = accumulate;
-    const int kcc = fft_kc(C);
-    if (E > 8) TR_FAIL(kUnsupported, "more than 8 echoes per adjoint launch");
+    const int kcc = fft_kc(C, p->ax[2].M);
     dim3 grid((unsigned)(nz * ny));
     ProfScope ps(kKAdjXCrop, 8.0 * K * nz * ny * gx + 8.0 * C * p->N + 8.0 * E * p->N * (accumulate ? 2 : 1), s);
-    if (int r = launch_adj_x(kcc, gx, grid, fft_smem(gx, kcc), s, a)) return r;
+    for (int e0 = 0; e0 < E; e0 += kAdjXEchoes) {  // at most kAdjXEchoes per launch
+      a.E = std::min(kAdjXEchoes, E - e0);
+      a.grid = p->work + (int64_t)e0 * C;
+      a.out = out + (int64_t)e0 * p->N;
+      if (int r = launch_adj_x(kcc, gx, grid, fft_smem(p->ax[2].M, kcc), s, a)) return r;
+    }
   }
   return kOk;
 }

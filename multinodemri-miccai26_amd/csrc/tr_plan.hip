@@ -15,7 +15,7 @@ namespace tr {
 // Plan construction kernels
 // ---------------------------------------------------------------------------
 __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh, int3 g, int3 nt,
-                             float4* kp, unsigned* key, int* idx) {
+                             float4* kp, unsigned long long* key, int* idx) {
   // k' = fl32(fl32(k * s) + shift), as SigPy's _scale_coord on a float32 trajectory:
   // the product must be rounded before the add (no FMA), or a coordinate that
   // SigPy puts exactly on a grid line (5 taps) lands 1 ulp off it (4 taps).
@@ -32,7 +32,8 @@ __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh,
   // Key = (kTile^3 tile of the first tap, first tap within the tile, 5-tap flags), so samples
   // with the same taps (same first tap and tap counts, as axis_taps computes them) are adjacent
   // in the sorted order and the interpolation can gather their taps once.
-  unsigned tile = 0, local = 0, five = 0;
+  unsigned long long tile = 0;
+  unsigned local = 0, five = 0;
   unsigned tq[3] = {0u, 0u, 0u};
   const float kv[3] = {kz, ky, kx};
   const int ntv[3] = {nt.x, nt.y, nt.z};
@@ -54,7 +55,7 @@ __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh,
   // blocks of the grid rather than rows
   for (int bit = 9; bit >= 0; --bit)
     tile = (tile << 3) | (((tq[0] >> bit) & 1u) << 2) | (((tq[1] >> bit) & 1u) << 1) | ((tq[2] >> bit) & 1u);
-  key[m] = (tile * (unsigned)((kTile + 2) * (kTile + 2) * (kTile + 2)) + local) * 8u + five;
+  key[m] = (tile * (unsigned long long)((kTile + 2) * (kTile + 2) * (kTile + 2)) + local) * 8ull + five;
   idx[m] = (int)m;
 }
 
@@ -380,11 +381,11 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   // up to 4 x 3 x 3 clusters per sample must index an int32 pair list
   if (M > (int64_t)17000000) TR_FAIL(kUnsupported, "more than 17M samples in one bin");
   float4* kp_raw = nullptr;
-  unsigned *key_in = nullptr, *key_out = nullptr;
+  unsigned long long *key_in = nullptr, *key_out = nullptr;
   int* idx = nullptr;
   TR_HIP(hipMalloc(&kp_raw, sizeof(float4) * M));
-  TR_HIP(hipMalloc(&key_in, sizeof(unsigned) * M));
-  TR_HIP(hipMalloc(&key_out, sizeof(unsigned) * M));
+  TR_HIP(hipMalloc(&key_in, sizeof(unsigned long long) * M));
+  TR_HIP(hipMalloc(&key_out, sizeof(unsigned long long) * M));
   TR_HIP(hipMalloc(&idx, sizeof(int) * M));
   TR_HIP(hipMalloc(&b.perm, sizeof(int) * M));
   TR_HIP(hipMalloc(&b.kp, sizeof(float4) * M));
@@ -405,7 +406,7 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   if (zspan > 1024) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
   while (((int64_t)1 << key_bits) < zspan * zspan * zspan * (kTile + 2) * (kTile + 2) * (kTile + 2) * 8)
     ++key_bits;
-  if (key_bits > 32) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
+  if (key_bits > 64) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
   size_t tmp_bytes = 0;
   TR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key_in, key_out, idx, b.perm, (int)M, 0,
                                             key_bits, s));
@@ -637,6 +638,10 @@ int build_cluster_order(tr_plan* p, BinData& b, int code) {
 
 void free_plan(tr_plan* p) {
   if (!p) return;
+  for (int i = 0; i < 3; ++i) {
+    (void)hipFree(p->chirp[i]);
+    (void)hipFree(p->hhat[i]);
+  }
   for (auto& b : p->bins) {
     (void)hipFree(b.perm);
     (void)hipFree(b.kp);
@@ -679,6 +684,54 @@ int check_plan_bin(const tr_plan* p, int bin) {
 
 using namespace tr;
 
+// Bluestein tables of axis i (length L, transform length M): chirp[n] = w_n =
+// exp(-i pi n^2 / L) for n < L, with n^2 reduced mod 2L in integers so the angle
+// is exact, and hhat = FFT_M(b) / M for b_m = conj(w_m), b_{M-m} = conj(w_m),
+// evaluated as a direct float64 DFT with integer-reduced angles.
+static int build_bluestein(tr_plan* p, int i) {
+  const int L = p->ax[i].L, M = p->ax[i].M;
+  std::vector<double> wr(L), wi(L);
+  for (int n = 0; n < L; ++n) {
+    const int64_t q = ((int64_t)n * n) % (2 * (int64_t)L);
+    const double a = -M_PI * (double)q / L;
+    wr[n] = std::cos(a);
+    wi[n] = std::sin(a);
+  }
+  std::vector<double> br(M, 0.0), bi(M, 0.0);
+  for (int m = 0; m < L; ++m) {
+    br[m] = wr[m];
+    bi[m] = -wi[m];
+    if (m > 0) {
+      br[M - m] = wr[m];
+      bi[M - m] = -wi[m];
+    }
+  }
+  std::vector<double> cr(M), ci(M);
+  for (int t = 0; t < M; ++t) {
+    cr[t] = std::cos(2.0 * M_PI * (double)t / M);
+    ci[t] = -std::sin(2.0 * M_PI * (double)t / M);
+  }
+  std::vector<float2> chirp(L), hhat(M);
+  for (int n = 0; n < L; ++n) chirp[n] = make_float2((float)wr[n], (float)wi[n]);
+  for (int k = 0; k < M; ++k) {
+    double sr = 0.0, si = 0.0;
+    for (int m = 0; m < M; ++m) {
+      if (br[m] == 0.0 && bi[m] == 0.0) continue;
+      const int t = (int)(((int64_t)m * k) % M);
+      sr += br[m] * cr[t] - bi[m] * ci[t];
+      si += br[m] * ci[t] + bi[m] * cr[t];
+    }
+    hhat[k] = make_float2((float)(sr / M), (float)(si / M));
+  }
+  TR_HIP(hipMalloc(&p->chirp[i], sizeof(float2) * L));
+  TR_HIP(hipMalloc(&p->hhat[i], sizeof(float2) * M));
+  TR_HIP(hipMemcpy(p->chirp[i], chirp.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+  TR_HIP(hipMemcpy(p->hhat[i], hhat.data(), sizeof(float2) * M, hipMemcpyHostToDevice));
+  p->ax[i].chirp = p->chirp[i];
+  p->ax[i].hhat = p->hhat[i];
+  return kOk;
+}
+
 extern "C" {
 
 int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
@@ -701,8 +754,9 @@ int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float
     p->coord_shift[i] = (float)(p->g[i] / 2);
     if (!make_fft_axis(p->g[i], &p->ax[i])) {
       free_plan(p);
-      TR_FAIL(kUnsupported, "oversampled grid length " + std::to_string(p->g[i]) +
-                                " is not supported (max 256, radices 2,3,5,7,11,13)");
+      TR_FAIL(kUnsupported, "oversampled grid length " + std::to_string(p->g[i]) + " is not supported (at most " +
+                                std::to_string(kMaxTransform) + " points with prime factors up to 13, or " +
+                                std::to_string(kMaxTransform / 2) + " otherwise)");
     }
   }
   p->G = (int64_t)p->g[0] * p->g[1] * p->g[2];
@@ -711,11 +765,17 @@ int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float
   p->ncl[2] = (p->g[2] + kClX - 1) / kClX;
   p->NCL = (int64_t)p->ncl[0] * p->ncl[1] * p->ncl[2];
   for (int i = 0; i < 3; ++i) {
-    const int L = p->g[i];
+    const int L = p->ax[i].M;  // the transform's twiddles
     std::vector<float2> tw(L);
     for (int t = 0; t < L; ++t) {
       const double a = -2.0 * M_PI * (double)t / L;
       tw[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    if (p->ax[i].bluestein && build_bluestein(p, i)) {
+      std::string msg = last_error();
+      free_plan(p);
+      set_error(msg);
+      return kOutOfMemory;
     }
     std::vector<float> ap(n[i]);
     for (int j = 0; j < n[i]; ++j) {

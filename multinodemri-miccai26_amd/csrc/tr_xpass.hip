@@ -15,9 +15,9 @@ template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
-  const int L = LC > 0 ? LC : a.ax.L;
+  const int LF = LC > 0 ? LC : a.ax.M;  // transform length (tile rows, twiddles)
   float2* tw = smem;
-  float2* buf = smem + L;
+  float2* buf = smem + LF;
   const int tid = threadIdx.x;
   const int ch = tid % KC, w = tid / KC;
   const int nz = a.gg.n[0], ny = a.gg.n[1], nx = a.gg.n[2];
@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
   const int64_t N = (int64_t)nz * ny * nx;
   const int c = c0 + ch;
   const bool live = c < a.C;
-  for (int i = tid; i < L; i += kBlock) tw[i] = a.tw[i];
+  for (int i = tid; i < LF; i += kBlock) tw[i] = a.tw[i];
   const int64_t rowvox = ((int64_t)z * ny + y) * nx;
   const float azy = a.apod_z[z] * a.apod_y[y] * a.scale;
   if constexpr (two_level_ok<KC, LC>()) {
@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
     // Every load below reads a clamped, in-bounds address and the padding is applied with a
     // select afterwards, so no load sits in a per-lane branch and all of them are in flight at once.
     // The E image rows (shared by the block's 16 coils) are staged in LDS behind the FFT tile.
-    const int E = a.K / a.C;
+    const int E = a.E;
     float2* img_row = buf + LC * T::ld;
     for (int i = tid; i < E * nx; i += kBlock) {
       const int e = i / nx;
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
     sm[it] = v;
   }
   const int zp = pmod(z - nz / 2, gz), yp = pmod(y - ny / 2, gy);
-  const int E = a.K / a.C;
+  const int E = a.E;
   for (int e = 0; e < E; ++e) {
     float2 u[T::PER];
 #pragma unroll
@@ -140,11 +140,11 @@ template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
-  constexpr int kSlots = (kMaxFftLen + kBlock - 1) / kBlock;
-  constexpr int kMaxE = 8;
-  const int L = LC > 0 ? LC : a.ax.L;
+  constexpr int kSlots = (max_line(KC) + kBlock - 1) / kBlock;
+  constexpr int kMaxE = kAdjXEchoes;
+  const int LF = LC > 0 ? LC : a.ax.M;  // transform length (tile rows, twiddles)
   float2* tw = smem;
-  float2* buf = smem + L;
+  float2* buf = smem + LF;
   const int tid = threadIdx.x;
   const int ch = tid % KC, w = tid / KC;
   const int nz = a.gg.n[0], ny = a.gg.n[1], nx = a.gg.n[2];
@@ -152,8 +152,8 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
   const int row = blockIdx.x;
   const int z = row / ny, y = row - (row / ny) * ny;
   const int64_t N = (int64_t)nz * ny * nx;
-  const int E = a.K / a.C;
-  for (int i = tid; i < L; i += kBlock) tw[i] = a.tw[i];
+  const int E = a.E;
+  for (int i = tid; i < LF; i += kBlock) tw[i] = a.tw[i];
   const int zp = pmod(z - nz / 2, gz), yp = pmod(y - ny / 2, gy);
   const float2* src = a.grid + (((int64_t)zp * gy + yp) * gx) * a.K;
   const int64_t rowvox = ((int64_t)z * ny + y) * nx;
@@ -227,59 +227,49 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
     }
     return;
   }
-  // Host guarantees E <= kMaxE.
-  float2 acc[kMaxE][kSlots];
+  // Echo-outer: one accumulator per image column slot of this thread; per (echo, column) the
+  // coils are added in increasing order (chunks in order, coils of a chunk in order), the same
+  // products in the same order as the two-level path.
+  for (int e = 0; e < E; ++e) {
+    float2 acc[kSlots];
 #pragma unroll
-  for (int e = 0; e < kMaxE; ++e)
+    for (int s = 0; s < kSlots; ++s) acc[s] = make_float2(0.f, 0.f);
+    for (int c0 = 0; c0 < a.C; c0 += KC) {
+      const int cn = min(KC, a.C - c0);
+      float2 v[T::PER];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) acc[e][s] = make_float2(0.f, 0.f);
-  for (int c0 = 0; c0 < a.C; c0 += KC) {
-    const int cn = min(KC, a.C - c0);
+      for (int it = 0; it < T::PER; ++it) {
+        const int p = w + it * T::W;
+        v[it] = (p < gx && ch < cn) ? src[(int64_t)p * a.K + (int64_t)e * a.C + c0 + ch] : make_float2(0.f, 0.f);
+      }
 #pragma unroll
-    for (int e = 0; e < kMaxE; ++e) {
-      if (e < E) {
-        float2 v[T::PER];
+      for (int it = 0; it < T::PER; ++it) {
+        const int p = w + it * T::W;
+        if (p < gx) buf[p * T::ld + ch] = cconj(v[it]);
+      }
+      __syncthreads();
+      fft_any<T::W, LC>(buf, T::ld, ch, w, a.ax, tw);
 #pragma unroll
-        for (int it = 0; it < T::PER; ++it) {
-          const int p = w + it * T::W;
-          v[it] = (p < gx && ch < cn) ? src[(int64_t)p * a.K + (int64_t)e * a.C + c0 + ch]
-                                       : make_float2(0.f, 0.f);
-        }
-#pragma unroll
-        for (int it = 0; it < T::PER; ++it) {
-          const int p = w + it * T::W;
-          if (p < gx) buf[p * T::ld + ch] = cconj(v[it]);
-        }
-        __syncthreads();
-        fft_any<T::W, LC>(buf, T::ld, ch, w, a.ax, tw);
-#pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
-          const int x = tid + s * kBlock;
-          if (x < nx) {
-            const int p = pmod(x - nx / 2, gx);
-            for (int cc = 0; cc < cn; ++cc) {
-              const float2 r = cconj(buf[p * T::ld + cc]);
-              acc[e][s] = cadd(acc[e][s],
-                               a.maps ? cmulc(a.maps[(int64_t)(c0 + cc) * N + rowvox + x], r) : r);
-            }
+      for (int s = 0; s < kSlots; ++s) {
+        const int x = tid + s * kBlock;
+        if (x < nx) {
+          const int p = pmod(x - nx / 2, gx);
+          for (int cc = 0; cc < cn; ++cc) {
+            const float2 r = cconj(buf[p * T::ld + cc]);
+            acc[s] = cadd(acc[s], a.maps ? cmulc(a.maps[(int64_t)(c0 + cc) * N + rowvox + x], r) : r);
           }
         }
-        __syncthreads();
       }
+      __syncthreads();
     }
-  }
 #pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    const int x = tid + s * kBlock;
-    if (x < nx) {
-      const float sc = (a.apod_z[z] * a.apod_y[y] * a.apod_x[x]) * a.scale;
-#pragma unroll
-      for (int e = 0; e < kMaxE; ++e) {
-        if (e < E) {
-          const float2 r = cscale(acc[e][s], sc);
-          float2* o = a.out + (int64_t)e * N + rowvox + x;
-          *o = a.accumulate ? cadd(*o, r) : r;
-        }
+    for (int s = 0; s < kSlots; ++s) {
+      const int x = tid + s * kBlock;
+      if (x < nx) {
+        const float sc = (a.apod_z[z] * a.apod_y[y] * a.apod_x[x]) * a.scale;
+        const float2 r = cscale(acc[s], sc);
+        float2* o = a.out + (int64_t)e * N + rowvox + x;
+        *o = a.accumulate ? cadd(*o, r) : r;
       }
     }
   }

@@ -21,7 +21,14 @@ def reference_grid(img):
     return np.fft.fftn(np.fft.ifftshift(x, axes=axes), axes=axes)  # natural order = fftshifted F
 
 
-@pytest.mark.parametrize("shape", [(12, 16, 16), (9, 11, 7), (8, 20, 6), (7, 9, 11), (16, 16, 24)])
+# grid lengths beyond the compile-time plans: 17, 47 and 31 are prime factors above 13 (Bluestein:
+# 170 = 2 5 17, 188 = 4 47, 1023 = 3 11 31, 513 = 27 19), 275 = 25 11, 320 and 2000 are smooth
+# lengths above 256 (smaller channel chunks)
+GENERAL = [(13, 9, 7), (136, 8, 8), (8, 150, 8), (8, 8, 220), (6, 256, 6), (4, 410, 4), (4, 4, 818),
+           (4, 4, 1600)]
+
+
+@pytest.mark.parametrize("shape", [(12, 16, 16), (9, 11, 7), (8, 20, 6), (7, 9, 11), (16, 16, 24)] + GENERAL)
 def test_forward_grid_matches_centred_fft(shape):
     import torch
 
@@ -43,7 +50,7 @@ def test_forward_grid_matches_centred_fft(shape):
     assert np.linalg.norm(hip - ref) / np.linalg.norm(ref) < 2e-6
 
 
-@pytest.mark.parametrize("shape", [(12, 16, 16), (9, 11, 7), (8, 20, 6)])
+@pytest.mark.parametrize("shape", [(12, 16, 16), (9, 11, 7), (8, 20, 6)] + GENERAL)
 def test_adjoint_grid_matches_centred_ifft(shape):
     import torch
 

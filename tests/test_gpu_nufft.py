@@ -179,7 +179,7 @@ def test_errors(torch_mod):
     with pytest.raises(ValueError, match="At least one trajectory bin"):
         HipNufftBackend((8, 8, 8), [], 0)
     with pytest.raises(_lib.HipLibraryError, match="not supported"):
-        HipNufftBackend((13, 8, 8), [_coords(rng, (13, 8, 8), 10)], 0)  # grid 17 = prime 17
+        HipNufftBackend((1700, 4, 4), [_coords(rng, (1700, 4, 4), 10)], 0)  # grid 2125: 17 | 2125 > 1024
 
 
 @pytest.mark.parametrize("shape,frac", [((12, 16, 20), 0.25), ((10, 14, 9), 0.4), ((16, 16, 16), 0.05)])
@@ -292,3 +292,66 @@ def test_spreading_orders_and_heavy_segments(torch_mod):
     with pytest.raises(Exception):
         plan.set_param("spread_order", 70000)
     plan.close()
+
+
+@pytest.mark.parametrize("shape", [(13, 9, 7), (136, 10, 8), (8, 150, 9), (9, 8, 220), (6, 256, 7)])
+def test_general_grid_lengths(torch_mod, shape):
+    """Grid lengths the compile-time FFT plans do not cover (17, 170, 188, 275 with prime factors
+    17, 47, 11 x 25; 320 above 256 points) against the oracle, forward and density-weighted adjoint
+    over 2 echoes x 3 coils."""
+    torch = torch_mod
+    from toporecon_hip import _lib
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    print("general lengths", shape, flush=True)
+    rng = np.random.default_rng(sum(shape))
+    coords = [_coords(rng, shape, 1200)]
+    be = HipNufftBackend(shape, coords, 0)
+    E, C = 2, 3
+    K = E * C
+    M = coords[0].shape[0]
+    img = _cplx(rng, (E, *shape))
+    maps = _cplx(rng, (C, *shape))
+    ti, tm = torch.from_numpy(img).cuda(), torch.from_numpy(maps).cuda()
+    out = torch.empty((K, M), dtype=torch.complex64, device="cuda")
+    s = _lib.stream_handle()
+    _lib.call("tr_nufft_forward", be.plan.handle, 0, ti.data_ptr(), tm.data_ptr(), E, C, out.data_ptr(), s)
+    got = out.cpu().numpy()
+    ref = np.stack([on.nufft(img[e].astype(np.complex128) * maps[c], coords[0]) for e in range(E) for c in range(C)])
+    assert _rel(got, ref) < TOL, _rel(got, ref)
+    q = _cplx(rng, (K, M))
+    dcf = rng.uniform(0.2, 1.0, M).astype(np.float32)
+    qt, dt = torch.from_numpy(q).cuda(), torch.from_numpy(dcf).cuda()  # held across the launch
+    g = torch.empty((E, *shape), dtype=torch.complex64, device="cuda")
+    _lib.call("tr_nufft_adjoint", be.plan.handle, 0, qt.data_ptr(), dt.data_ptr(), tm.data_ptr(), E, C,
+              g.data_ptr(), 0, s)
+    refa = np.stack([sum(np.conj(maps[c].astype(np.complex128)) * on.nufft_adjoint(q[e * C + c] * dcf, coords[0], shape)
+                         for c in range(C)) for e in range(E)])
+    assert _rel(g.cpu().numpy(), refa) < TOL, _rel(g.cpu().numpy(), refa)
+
+
+def test_many_echoes_and_long_rows(torch_mod):
+    """12 echoes: the inverse x pass runs in groups of 8 echoes and, for 600-voxel rows, the forward
+    x pass stages the echoes' image rows in groups that fit its LDS; both match the oracle."""
+    torch = torch_mod
+    from toporecon_hip import _lib
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    print("many echoes", flush=True)
+    for shape in ((6, 8, 10), (4, 5, 600)):
+        rng = np.random.default_rng(31)
+        coords = [_coords(rng, shape, 700)]
+        be = HipNufftBackend(shape, coords, 0)
+        E, C = 12, 1
+        M = coords[0].shape[0]
+        img = _cplx(rng, (E, *shape))
+        out = torch.empty((E, M), dtype=torch.complex64, device="cuda")
+        s = _lib.stream_handle()
+        it = torch.from_numpy(img).cuda()  # held across the launch
+        _lib.call("tr_nufft_forward", be.plan.handle, 0, it.data_ptr(), None, E, C, out.data_ptr(), s)
+        ref = np.stack([on.nufft(img[e].astype(np.complex128), coords[0]) for e in range(E)])
+        assert _rel(out.cpu().numpy(), ref) < TOL
+        q = _cplx(rng, (E, M))
+        qt = torch.from_numpy(q).cuda()
+        g = torch.empty((E, *shape), dtype=torch.complex64, device="cuda")
+        _lib.call("tr_nufft_adjoint", be.plan.handle, 0, qt.data_ptr(), None, None, E, C, g.data_ptr(), 0, s)
+        refa = np.stack([on.nufft_adjoint(q[e].astype(np.complex128), coords[0], shape) for e in range(E)])
+        assert _rel(g.cpu().numpy(), refa) < TOL
