@@ -126,3 +126,39 @@ def test_tvme_80x192x192_full_bin_properties(torch_mod):
         assert torch.equal(torch.view_as_real(fwd(x1)), torch.view_as_real(a1))
     finally:
         s.plan.set_param("prune", 1)
+
+
+def test_tvm_48x64x64_full_solve_matches_oracle(torch_mod):
+    """The reference's CPU-runnable configuration solved to its stopping rule (lambda 1e-5, tol 1e-3,
+    at most 100 iterations) against the float64 oracle's solve frozen in tests/golden/tvm48_solve.npz
+    (tests/golden/make_tvm48_golden.py): both run the 100-iteration cap, the stopping values agree
+    iteration by iteration, and the final image agrees to 1e-4 on 20,000 seeded voxels, in norm and
+    in 4 seeded projections (stopping values to 1e-2: they are ratios of small differences)."""
+    import os
+
+    torch = torch_mod
+    from toporecon_hip import synth
+    from toporecon_hip.algorithms import TvmReconstructor
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    gold = np.load(os.path.join(here, "tvm48_solve.npz"))
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_tvm48_golden", os.path.join(here, "make_tvm48_golden.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    print("tvm 48x64x64 full solve", flush=True)
+    cfg = synth.config_small_tvm()
+    acq = synth.make_acquisition(cfg)
+    ksp = synth.simulate_kspace_numpy(acq, oracle_forward)
+    B = len(acq.bin_edges) - 1
+    s = TvmReconstructor(ksp, acq.coord, acq.dcf, acq.mps, acq.resp, None, B, cfg.l2_coupling,
+                         lamda=cfg.lambda_motion, sigma=1 / 6, tau=1 / 6, max_iter=cfg.max_iter, tol=cfg.tol,
+                         margin=0, device=0, bin_edges=acq.bin_edges, show_pbar=False)
+    ub = s.run().cpu().numpy().astype(np.complex128)
+    assert s.iterations == int(gold["iterations"]) == 100
+    hist = np.asarray(s.history)
+    assert np.max(np.abs(hist - gold["history"]) / gold["history"]) < 1e-2
+    assert tuple(ub.shape) == tuple(gold["shape"])
+    assert rel(ub.reshape(-1)[gold["voxel_index"]], gold["voxels"]) < 1e-4
+    assert abs(np.linalg.norm(ub) - float(gold["norm"])) / float(gold["norm"]) < 1e-5
+    proj = np.array([np.vdot(mk.projection_volume(sd, ub.shape), ub) for sd in mk.PROJ_SEEDS])
+    assert np.max(np.abs(proj - gold["projections"]) / np.abs(gold["projections"])) < 1e-3
