@@ -257,6 +257,36 @@ int tr_cnn_sumsq(const tr_c64* img, int64_t V, int n, double* out, void* stream)
 int tr_data_fit_sums(const tr_c64* a_sorted, const tr_c64* y_sorted, const float* dcf_sorted, int64_t M, int K,
                      double* work, int nblocks, double* out, void* stream);
 
+/* ---- JSENSE coil-sensitivity estimation ----
+ * Replace the linear operators of SigPy 0.1.27's sigpy.mri.app.JsenseRecon, which the reference
+ * runs on the first echo (src/toporecon/preprocessing/jsense.py:91).  Its operators are
+ * sqrt(w) NUFFT(IFFT(mps_ker_c (*) img_ker)) (sigpy.mri.linop ConvSense / ConvImage); the NUFFT is
+ * a plan of this library (tr_data_simulate / tr_data_adjoint), the rest is below.
+ * (*) is SigPy's 'valid' convolution (sp.linop.ConvolveInput / ConvolveFilter, mode 'valid',
+ * output_multi_channel): img (iz, iy, ix), ker (C, W, W, W), out (C, iz-W+1, iy-W+1, ix-W+1),
+ * out[c, n] = sum_k ker[c, k] img[n + W - 1 - k].  W <= 64, W <= each dim <= 1024.
+ * work holds tr_js_work_count() complex values (partial sums, added in a fixed order). */
+int64_t tr_js_work_count(int iz, int iy, int ix, int C, int W);
+int tr_js_conv(const tr_c64* img, int iz, int iy, int ix, const tr_c64* ker, int C, int W, tr_c64* out,
+               tr_c64* work, void* stream);
+/* Adjoint in the image kernel: out (iz, iy, ix) = sum_c conv^H(z_c; ker_c), z (C, iz-W+1, ..). */
+int tr_js_conv_adj_img(const tr_c64* z, int iz, int iy, int ix, const tr_c64* ker, int C, int W, tr_c64* out,
+                       tr_c64* work, void* stream);
+/* Adjoint in the map kernel: out (C, W, W, W), out[c, k] = sum_n z[c, n] conj(img[n + W - 1 - k]). */
+int tr_js_conv_adj_ker(const tr_c64* z, const tr_c64* img, int iz, int iy, int ix, int C, int W, tr_c64* out,
+                       tr_c64* work, void* stream);
+/* out[o, j, t] = sum_i mat[j, i] x[o, i, t] for x (outer, n_in, inner), mat (n_out, n_in) on the device,
+ * out (outer, n_out, inner), not aliasing x: one axis of sp.fft / sp.ifft (centred, orthonormal) on
+ * the calibration grid, and of the k-space resizes of JsenseRecon's output and of
+ * jsense.py:93-96 (ifft(resize(fft(map)))). */
+int tr_js_axis_apply(const tr_c64* x, int64_t outer, int n_in, int64_t inner, const tr_c64* mat, int n_out,
+                     tr_c64* out, void* stream);
+/* maps (C, N): each voxel divided by the root sum of squares over the coils (no floor), as
+ * JsenseRecon's output step. */
+int tr_js_rss_normalize(tr_c64* maps, int C, int64_t N, void* stream);
+/* y += a x (n complex values): the lamda * I term of JsenseRecon's regularised least squares. */
+int tr_js_axpy(tr_c64* y, const tr_c64* x, int64_t n, float a, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

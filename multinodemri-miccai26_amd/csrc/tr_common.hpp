@@ -66,6 +66,11 @@ enum KernelId : int {
   kKW2Dual,
   kKTvmwDual,
   kKTvmwPrimal,
+  kKJsConv,
+  kKJsConvAdjImg,
+  kKJsConvAdjKer,
+  kKJsAxis,
+  kKJsRss,
   kNumKernelIds
 };
 

@@ -55,7 +55,8 @@ const char* kernel_name(int id) {
   static const char* names[kNumKernelIds] = {
       "fwd_x_pad", "fft_y_fwd", "fft_z_fwd", "interp", "spread", "fft_z_inv", "fft_y_inv",
       "adj_x_crop", "tvm_dual", "tvm_primal", "tvme_dual", "tvme_primal", "reduce",
-      "w2_forward", "w2_adjoint", "w2_dual", "tvmw_dual", "tvmw_primal"};
+      "w2_forward", "w2_adjoint", "w2_dual", "tvmw_dual", "tvmw_primal", "js_conv", "js_conv_adj_img",
+      "js_conv_adj_ker", "js_axis", "js_rss"};
   return (id >= 0 && id < kNumKernelIds) ? names[id] : "";
 }
 

@@ -79,6 +79,13 @@ SIGNATURES = {
     "tr_cnn_unpack": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp]),
     "tr_cnn_sumsq": (c_int, [c_vp, c_i64, c_int, c_vp, c_vp]),
     "tr_data_fit_sums": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp]),
+    "tr_js_work_count": (c_i64, [c_int, c_int, c_int, c_int, c_int]),
+    "tr_js_conv": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
+    "tr_js_conv_adj_img": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
+    "tr_js_conv_adj_ker": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "tr_js_axis_apply": (c_int, [c_vp, c_i64, c_int, c_i64, c_vp, c_int, c_vp, c_vp]),
+    "tr_js_rss_normalize": (c_int, [c_vp, c_int, c_i64, c_vp]),
+    "tr_js_axpy": (c_int, [c_vp, c_vp, c_i64, c_f32, c_vp]),
 }
 
 _lib = None
