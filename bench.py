@@ -70,11 +70,12 @@ def parse_args(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=("full", "small", "batched", "tvm48", "cnn"), default="full",
+    ap.add_argument("--config", choices=("full", "small", "batched", "tvm48", "cnn", "jsense"), default="full",
                     help="full: TVME 80x192x192 (the headline); small: a reduced TVME; batched: 12 uneven "
                          "TVM instances, HIP-graph batch vs sequential solves; tvm48: plain TVM at "
                          "48x64x64, 8 coils, 2 echoes, 4 bins beside one full single-precision CPU "
-                         "iteration; cnn: the warm-start network's MFMA convolutions")
+                         "iteration; cnn: the warm-start network's MFMA convolutions; jsense: coil-map "
+                         "estimation on echo 0 of the headline acquisition")
     ap.add_argument("--max-iter", type=int, default=None, help="override max_iter (profiling runs)")
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--budget-s", type=float, default=DEFAULT_BUDGET_S,
@@ -520,6 +521,207 @@ def _tvm48_cpu_iteration():
     s32.tvm_iteration(prob, st, cfg.lambda_motion, cfg.l2_coupling)
     print(json.dumps({"per_iter": time.perf_counter() - t0, "bins": prob.B, "K": prob.E * prob.C,
                       "samples": [int(k.shape[-1]) for k in prob.ksp]}))
+
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X published dense fp32 peak
+
+
+def _jsense_cpu_sample(job_path):
+    """Time the float64 JSENSE oracle's pieces on a calibration problem of the benchmark's shape
+    (random values; the cost depends on shapes only); prints one JSON line."""
+    import numpy as np
+
+    from oracle import jsense as oj
+    from oracle import nufft as on
+    with open(job_path) as f:
+        job = json.load(f)
+    if job.get("core") is not None:
+        os.sched_setaffinity(0, {int(job["core"])})
+    rng = np.random.default_rng(0)
+    C, W, grd, M, frac = job["C"], job["W"], tuple(job["grd"]), job["M"], job["nufft_fraction"]
+    I = tuple(g + W - 1 for g in grd)
+
+    def cr(*shape):
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    img, ker, z = cr(*I), cr(C, W, W, W), cr(C, *grd)
+    t = {}
+    t0 = time.perf_counter()
+    oj.conv_valid(img[None], ker)
+    t["conv"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oj.conv_valid_adj_img(z, ker, I)
+    t["conv_adj_img"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oj.conv_valid_adj_ker(z, img, ker.shape)
+    t["conv_adj_ker"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oj.ifft_c(oj.fft_c(z))
+    t["fft_pair"] = time.perf_counter() - t0
+    ms = max(1, int(M * frac))
+    coord = rng.uniform(-11.9, 11.9, (ms, 3)).astype(np.float32)
+    t0 = time.perf_counter()
+    k = on.nufft(z, coord)
+    t["nufft"] = (time.perf_counter() - t0) * M / ms
+    t0 = time.perf_counter()
+    on.nufft_adjoint(k, coord, grd)
+    t["nufft_adjoint"] = (time.perf_counter() - t0) * M / ms
+    full = cr(*job["img_shape"])
+    t0 = time.perf_counter()
+    oj.ifft_c(full)
+    t["map_fft"] = time.perf_counter() - t0
+    print(json.dumps({"t": t, "nufft_samples_timed": ms}))
+
+
+def bench_jsense(args, verbose=True):
+    """JSENSE coil-map estimation at the benchmark's acquisition (80x192x192, 32 coils, 9000 x 512 cone
+    samples of echo 0, readout fraction 0.95, legacy_max coordinates) with SigPy's JsenseRecon defaults
+    (map kernels 16^3, calibration width 24, 10 x 10 CG steps).  A step is one estimation from
+    calibration data resident in HBM: the alternating minimisation and the output maps at the image
+    shape (JsenseHip.run + maps); the host part (calibration selection, upload, plan) and the full
+    host-to-host call are reported beside it."""
+    import numpy as np
+    import torch
+
+    from toporecon_hip import _lib, synth
+    from toporecon_hip.data.trajectory import prepare_trajectory
+    from toporecon_hip.jsense import JsenseHip
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    dev = torch.device("cuda", 0)
+    cfg = synth.config_full_tvme()
+    shape = tuple(cfg.img_shape)
+    coord = synth.cone_trajectory(cfg.interleaves, cfg.readouts, shape, cfg.turns, cfg.cone_angle)
+    raw_xyz = np.ascontiguousarray(coord[..., ::-1] / np.asarray(shape[::-1], np.float32))
+    traj = prepare_trajectory(raw_xyz, synth.density(coord), image_shape_zyx=shape, voxel_size_zyx=(1, 1, 1),
+                              readout_fraction=0.95, scaling="legacy_max")
+    c = traj.coordinates_zyx
+    maps = synth.coil_maps(cfg.coils, shape, cfg.seed)
+    truth = synth.phantom(shape, cfg.echo_times_ms[:1], 1, cfg.seed, cfg.ellipsoids)[0]
+    be = HipNufftBackend(shape, [c], dev)
+    ksp = be.forward_batch(torch.from_numpy(truth).to(dev), torch.from_numpy(maps).to(dev), 0)
+    ksp = ksp.reshape(1, cfg.coils, *c.shape[:2]).cpu().numpy()
+    be.plan.close()
+    del be
+    log(f"jsense: k-space {ksp.shape}, trajectory {c.shape}", verbose)
+
+    def one(profile=False):
+        t0 = time.perf_counter()
+        s = JsenseHip(ksp[0], c, traj.density, dev)
+        torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        if profile:
+            _lib.profile_enable(True)
+        s.run()
+        m = s.maps(shape)
+        torch.cuda.synchronize(dev)
+        t2 = time.perf_counter()
+        prof = _lib.profile_read() if profile else None
+        if profile:
+            _lib.profile_enable(False)
+        host = m.cpu().numpy()
+        t3 = time.perf_counter()
+        info = {"M": s.M, "grd": list(s.grd_shape), "img_shape": list(s.img_shape), "C": s.C, "W": s.W,
+                "objective": s.objective(), "max_iter": s.max_iter, "max_inner_iter": s.max_inner_iter}
+        s.close()
+        return t1 - t0, t2 - t1, t3 - t0, prof, info, host
+
+    for _ in range(max(args.warmup, 1)):
+        one()
+    dev_t, host_t, e2e_t = [], [], []
+    host_maps = None
+    for _ in range(args.steps):
+        th, td, te, _, info, host_maps = one()
+        dev_t.append(td)
+        host_t.append(th)
+        e2e_t.append(te)
+    _, _, _, prof, info, m_prof = one(profile=True)
+    assert np.array_equal(m_prof, host_maps), "JSENSE maps differ between runs"
+    t_dev = sum(dev_t) / len(dev_t)
+    log(f"jsense: device {t_dev:.3f} s, host prep {sum(host_t) / len(host_t):.3f} s, "
+        f"host-to-host {sum(e2e_t) / len(e2e_t):.3f} s; calibration grid {info['grd']}, {info['M']} samples",
+        verbose)
+    C, W = info["C"], info["W"]
+    Gn = math.prod(info["grd"])
+    conv_flops = 8.0 * C * Gn * W ** 3
+    kernels = {}
+    for name, (ms, n, by) in prof.items():
+        e = {"launches": n, "ms_avg": ms / n, "ms_total": ms}
+        if name.startswith("js_conv"):
+            e["tflops"] = conv_flops / (ms / n * 1e-3) / 1e12
+        kernels[name] = e
+    dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+    dk = kernels[dom]
+    if dom.startswith("js_conv"):
+        roofline = {"bound": "fp32", "kernel": dom, "achieved": dk["tflops"], "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": dk["tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
+                    "flops_model": "8 C G W^3 per launch (a complex multiply-add per coil, calibration-grid "
+                                   "point and tap; C coils, G calibration-grid points, W^3 taps)",
+                    "algorithmic_flops_per_launch": conv_flops, "avg_launch_ms": dk["ms_avg"],
+                    "launches": dk["launches"],
+                    "note": "FP32 FMA on the vector ALUs without packed FP32, whose ceiling is half the "
+                            "published fp32 peak used here"}
+    else:
+        # the calibration NUFFT: C coil images (no maps), one bin of M samples on the small grid
+        g = [math.ceil(1.25 * v) for v in info["grd"]]
+        model = model_bytes_per_launch(dom, info["M"], C, 1, Gn, math.prod(g), 1)
+        achieved = model / (dk["ms_avg"] * 1e-3) / 1e9 if model else None
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBPS if achieved else None, "traffic": None,
+                    "algorithmic_bytes_per_launch": model, "avg_launch_ms": dk["ms_avg"],
+                    "launches": dk["launches"],
+                    "note": f"calibration NUFFT: K = {C} channels, M = {info['M']} samples on a "
+                            f"{'x'.join(map(str, g))} grid; every sample's 64 taps land in a grid that fits in "
+                            f"cache, so the kernel is bound by the per-tap work, not by HBM"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        core = sorted(os.sched_getaffinity(0))[0]
+        job = {"core": core, "C": C, "W": W, "grd": info["grd"], "M": info["M"], "nufft_fraction": 0.125,
+               "img_shape": info["img_shape"]}
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "jsense_job.json")
+            with open(path, "w") as f:
+                json.dump(job, f)
+            log("cpu baseline: float64 JSENSE oracle pieces on 1 pinned core", verbose)
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker",
+                                "jsense:" + path], capture_output=True, text=True, env=_cpu_env(), timeout=900)
+        if r.returncode == 0:
+            d = json.loads(r.stdout.strip().splitlines()[-1])["t"]
+            data = d["nufft"] + d["nufft_adjoint"] + d["fft_pair"]
+            n_mps = info["max_inner_iter"] + 1  # SigPy's CG applies the normal operator once before its steps
+            per_round = (data + d["conv_adj_ker"] + n_mps * (d["conv"] + data + d["conv_adj_ker"])
+                         + data + d["conv_adj_img"] + n_mps * (d["conv"] + data + d["conv_adj_img"]))
+            total = info["max_iter"] * per_round + 3 * C * d["map_fft"]
+            cpu = {"value": total, "unit": "s", "cores": 1, "kind": "port", "cpu_model": _cpu_model(),
+                   "pieces_s": d,
+                   "sample": ("the float64 oracle (oracle/jsense.py, oracle/nufft.py) on 1 pinned thread: one "
+                              "valid convolution and each adjoint at the full shape, the centred FFT pair, the "
+                              "NUFFT pair on 1/8 of the calibration samples (x8), one map-size inverse FFT; "
+                              "projected to SigPy's schedule: per round two least-squares solves, each A^H y "
+                              "plus (inner + 1) normal-operator applications, then 3 map-size FFTs per coil")}
+        else:
+            log(f"cpu baseline failed: {r.stderr[-2000:]}", True)
+    line = {
+        "metric": "JSENSE coil-map estimation time (SigPy JsenseRecon defaults) on 1 GPU",
+        "value": t_dev, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_dev * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded cone trajectory, ellipsoid phantom, Gaussian-blob coil maps, echo-0 k-space "
+                "simulated by the HIP forward NUFFT (no noise)",
+        "config": {"workload": (f"JSENSE on echo 0 of {'x'.join(map(str, shape))}, {cfg.coils} coils, "
+                                f"{cfg.interleaves}x{cfg.readouts} cone samples (readout fraction 0.95, legacy_max "
+                                f"coordinates): map kernels {W}^3, calibration width 24, "
+                                f"{info['max_iter']} x {info['max_inner_iter']} CG steps"),
+                   "step": "JsenseHip.run + maps at the image shape, calibration data resident in HBM",
+                   "calibration_grid": info["grd"], "calibration_samples": info["M"],
+                   "host_prep_s": sum(host_t) / len(host_t),
+                   "host_to_host_s": sum(e2e_t) / len(e2e_t),
+                   "final_objective": info["objective"],
+                   "run_wall_s": time.perf_counter() - T_PROCESS0},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def bench_tvm48(args, verbose=True):
@@ -970,7 +1172,10 @@ def bench_full(args, verbose=True):
 def main(argv=None):
     args = parse_args(argv)
     if args.cpu_baseline_worker:
-        _cpu_worker(args.cpu_baseline_worker)
+        if args.cpu_baseline_worker.startswith("jsense:"):
+            _jsense_cpu_sample(args.cpu_baseline_worker[len("jsense:"):])
+        else:
+            _cpu_worker(args.cpu_baseline_worker)
         return 0
     if args.traffic_from:
         out = write_traffic(args.traffic_from[0], args.traffic_from[1], args.copy_calibration_mib or 2048,
@@ -983,6 +1188,8 @@ def main(argv=None):
         return 2
     if args.config == "cnn":
         return bench_cnn(args)
+    if args.config == "jsense":
+        return bench_jsense(args)
     if args.config == "tvm48":
         return bench_tvm48(args)
     if args.config == "batched":

@@ -89,28 +89,48 @@ __global__ __launch_bounds__(kJsBlock) void k_js_conv(JsConv g, const float2* __
   for (int c = 0; c < CPT; ++c) part[((int64_t)slab * g.C + c0 + c) * g.Gn + n] = acc[c];
 }
 
-// Adjoint in the image kernel, coils [c0, c0 + cpg) of group blockIdx.y:
-// part[grp * In + i] = sum_c sum_k conj(ker[c, k]) z[c, i - (W - 1) + k] over the valid k.
-// The tap loops run over the full range with a per-lane predicate, so every tap index is uniform and
-// a plane no lane of the wave needs is skipped as a whole.
-__global__ __launch_bounds__(kJsBlock) void k_js_conv_adj_img(JsConv g, const float2* __restrict__ z,
+// Rows of z padded with W - 1 zeros on both sides in x (and kJsAdjRows - 1 more at the end), so the
+// image-kernel adjoint reads whole tap windows without bounds checks: zx[c, nz, ny, p] =
+// z[c, nz, ny, p - (W - 1)] inside, 0 outside; rows of px values.
+__global__ __launch_bounds__(kJsBlock) void k_js_pad_x(JsConv g, const float2* __restrict__ z,
+                                                      float2* __restrict__ zx, int px) {
+  const int64_t n = (int64_t)g.C * g.Gz * g.Gy * px;
+  const int64_t stride = (int64_t)gridDim.x * kJsBlock;
+  for (int64_t j = (int64_t)blockIdx.x * kJsBlock + threadIdx.x; j < n; j += stride) {
+    const int p = (int)(j % px);
+    const int64_t row = j / px;
+    const int x = p - (g.W - 1);
+    zx[j] = (x >= 0 && x < g.Gx) ? z[row * g.Gx + x] : make_float2(0.f, 0.f);
+  }
+}
+
+// Adjoint in the image kernel, coils [c0, c0 + cpg) of group blockIdx.y, R consecutive outputs in x
+// per thread: part[grp * In + i] = sum_c sum_k conj(ker[c, k]) z[c, i - (W - 1) + k] over the valid
+// (kz, ky); every kx runs against the x-padded rows.  With the width known at compile time (W_T > 0)
+// a thread loads its R + W - 1 window values of a row once and reuses them for R * W products.
+// The (kz, ky) loops run over the full range with a per-lane predicate, so the tap indices are
+// uniform (kernel values load through the scalar path) and a plane no lane needs is skipped.
+template <int W_T, int R>
+__global__ __launch_bounds__(kJsBlock) void k_js_conv_adj_img(JsConv g, const float2* __restrict__ zx, int px,
                                                              const float2* __restrict__ ker,
-                                                             float2* __restrict__ part, int cpg) {
-  const int64_t i = (int64_t)blockIdx.x * kJsBlock + threadIdx.x;
-  if (i >= g.In) return;
+                                                             float2* __restrict__ part, int cpg, int nxg) {
+  const int64_t t = (int64_t)blockIdx.x * kJsBlock + threadIdx.x;
+  if (t >= (int64_t)g.Iz * g.Iy * nxg) return;
+  const int W = W_T > 0 ? W_T : g.W;
+  const int xg = (int)(t % nxg);
+  const int64_t row = t / nxg;
+  const int iy = (int)(row % g.Iy);
+  const int iz = (int)(row / g.Iy);
+  const int ix0 = xg * R;
   const int grp = blockIdx.y;
   const int c0 = grp * cpg;
   const int c1 = min(g.C, c0 + cpg);
-  const int ix = (int)(i % g.Ix);
-  const int64_t t = i / g.Ix;
-  const int iy = (int)(t % g.Iy);
-  const int iz = (int)(t / g.Iy);
-  const int W = g.W;
   // valid taps: 0 <= i - (W - 1) + k < G per axis
   const int zlo = max(0, W - 1 - iz), zhi = min(W - 1, W - 1 + g.Gz - 1 - iz);
   const int ylo = max(0, W - 1 - iy), yhi = min(W - 1, W - 1 + g.Gy - 1 - iy);
-  const int xlo = max(0, W - 1 - ix), xhi = min(W - 1, W - 1 + g.Gx - 1 - ix);
-  float2 acc = make_float2(0.f, 0.f);
+  float2 acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = make_float2(0.f, 0.f);
   for (int c = c0; c < c1; ++c) {
     for (int kz = 0; kz < W; ++kz) {
       if (kz < zlo || kz > zhi) continue;
@@ -118,15 +138,33 @@ __global__ __launch_bounds__(kJsBlock) void k_js_conv_adj_img(JsConv g, const fl
       for (int ky = 0; ky < W; ++ky) {
         if (ky < ylo || ky > yhi) continue;
         const int ny = iy - (W - 1) + ky;
-        const float2* zr = z + (((int64_t)c * g.Gz + nz) * g.Gy + ny) * g.Gx + (ix - (W - 1));
+        const float2* zr = zx + (((int64_t)c * g.Gz + nz) * g.Gy + ny) * px + ix0;
         const float2* kr = ker + (int64_t)c * g.W3 + (int64_t)(kz * W + ky) * W;
-        for (int kx = 0; kx < W; ++kx) {
-          if (kx >= xlo && kx <= xhi) acc = cfma_conj(kr[kx], zr[kx], acc);
+        if constexpr (W_T > 0) {
+          float2 w[R + W_T - 1];
+#pragma unroll
+          for (int q = 0; q < R + W_T - 1; ++q) w[q] = zr[q];
+#pragma unroll
+          for (int kx = 0; kx < W_T; ++kx) {
+            const float2 kv = kr[kx];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = cfma_conj(kv, w[r + kx], acc[r]);
+          }
+        } else {
+          for (int kx = 0; kx < W; ++kx) {
+            const float2 kv = kr[kx];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = cfma_conj(kv, zr[r + kx], acc[r]);
+          }
         }
       }
     }
   }
-  part[(int64_t)grp * g.In + i] = acc;
+  float2* out = part + (int64_t)grp * g.In + ((int64_t)iz * g.Iy + iy) * g.Ix;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (ix0 + r < g.Ix) out[ix0 + r] = acc[r];
+  }
 }
 
 // Adjoint in the map kernel, sample planes [nz0, nz1) of slab blockIdx.z, CPT coils per thread:
@@ -297,22 +335,38 @@ ConvLaunch adj_ker_launch(const JsConv& g) {
   return l;
 }
 
-// coils per group of the image-kernel adjoint
-int adj_img_cpg(const JsConv& g) {
-  const int64_t base = div_up(g.In, kJsBlock);
+// Image-kernel adjoint launch: outputs per thread in x, the x-padded row length, threads per coil
+// group, coils per group.
+constexpr int kJsAdjRows = 4;
+constexpr int kJsAdjWidth = 16;  // the map-kernel width with a compile-time window (SigPy's default)
+
+struct AdjImgLaunch {
+  int R, px, nxg, cpg, groups;
+  int64_t threads;
+};
+
+AdjImgLaunch adj_img_launch(const JsConv& g) {
+  AdjImgLaunch l;
+  l.R = g.W == kJsAdjWidth ? kJsAdjRows : 1;
+  l.px = g.Gx + 2 * (g.W - 1) + l.R - 1;
+  l.nxg = div_up(g.Ix, l.R);
+  l.threads = (int64_t)g.Iz * g.Iy * l.nxg;
+  const int64_t base = div_up(l.threads, kJsBlock);
   int groups = div_up(kJsTargetBlocks, base);
   if (groups < 1) groups = 1;
   if (groups > g.C) groups = g.C;
-  return div_up(g.C, groups);
+  l.cpg = div_up(g.C, groups);
+  l.groups = div_up(g.C, l.cpg);
+  return l;
 }
 
 int64_t work_count(const JsConv& g) {
   const ConvLaunch f = fwd_launch(g), a = adj_ker_launch(g);
-  const int64_t groups = div_up(g.C, adj_img_cpg(g));
+  const AdjImgLaunch m = adj_img_launch(g);
   int64_t n = (int64_t)f.slabs * g.C * g.Gn;
   n = n > (int64_t)a.slabs * g.C * g.W3 ? n : (int64_t)a.slabs * g.C * g.W3;
-  n = n > groups * g.In ? n : groups * g.In;
-  return n;
+  const int64_t ni = m.groups * g.In + (int64_t)g.C * g.Gz * g.Gy * m.px;
+  return n > ni ? n : ni;
 }
 
 unsigned ew_blocks(int64_t n) {
@@ -368,14 +422,22 @@ int tr_js_conv_adj_img(const tr_c64* z, int iz, int iy, int ix, const tr_c64* ke
   if (int r = make_conv(iz, iy, ix, C, W, &g)) return r;
   if (!z || !ker || !out || !work) TR_FAIL(kInvalidArgument, "null argument");
   hipStream_t s = (hipStream_t)stream;
-  const int cpg = adj_img_cpg(g);
-  const int groups = div_up(g.C, cpg);
+  const AdjImgLaunch l = adj_img_launch(g);
+  float2* part = (float2*)work;
+  float2* zx = part + (int64_t)l.groups * g.In;
   ProfScope ps(kKJsConvAdjImg, 8.0 * (g.C * g.Gn + g.C * g.W3 + g.In), s);
-  const dim3 grid(div_up(g.In, kJsBlock), groups, 1);
-  hipLaunchKernelGGL(k_js_conv_adj_img, grid, dim3(kJsBlock), 0, s, g, (const float2*)z, (const float2*)ker,
-                     (float2*)work, cpg);
+  hipLaunchKernelGGL(k_js_pad_x, dim3(ew_blocks((int64_t)g.C * g.Gz * g.Gy * l.px)), dim3(kJsBlock), 0, s, g,
+                     (const float2*)z, zx, l.px);
   TR_LAUNCH_CHECK();
-  return sum_slabs((const float2*)work, groups, g.In, (float2*)out, s);
+  const dim3 grid(div_up(l.threads, kJsBlock), l.groups, 1);
+  if (l.R == kJsAdjRows)
+    hipLaunchKernelGGL((k_js_conv_adj_img<kJsAdjWidth, kJsAdjRows>), grid, dim3(kJsBlock), 0, s, g,
+                       (const float2*)zx, l.px, (const float2*)ker, part, l.cpg, l.nxg);
+  else
+    hipLaunchKernelGGL((k_js_conv_adj_img<0, 1>), grid, dim3(kJsBlock), 0, s, g, (const float2*)zx, l.px,
+                       (const float2*)ker, part, l.cpg, l.nxg);
+  TR_LAUNCH_CHECK();
+  return sum_slabs(part, l.groups, g.In, (float2*)out, s);
 }
 
 int tr_js_conv_adj_ker(const tr_c64* z, const tr_c64* img, int iz, int iy, int ix, int C, int W, tr_c64* out,

@@ -264,8 +264,9 @@ def tvmw_main(argv=None) -> int:
 
 
 def cli_main(argv=None) -> int:
-    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS`` and ``resp INPUT --work-dir DIR`` as in
-    src/toporecon/cli.py:75-183.  The shards are the reference's format, so its own ``toporecon
+    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS``, ``resp INPUT --work-dir DIR`` and ``jsense
+    INPUT --work-dir DIR [--device --readout-fraction --coordinate-scaling --fov-scale --show-progress]``
+    as in src/toporecon/cli.py:23-57, :75-88, :135-196.  The shards are the reference's format, so its own ``toporecon
     stitch`` assembles them."""
     p = argparse.ArgumentParser(prog="toporecon_hip")
     p.add_argument("--verbose", action="store_true")
@@ -276,6 +277,17 @@ def cli_main(argv=None) -> int:
     rp = sub.add_parser("resp", help="Estimate resp.hdr/.cfl in a separate work directory.")
     rp.add_argument("input", type=Path)
     rp.add_argument("--work-dir", type=Path, required=True)
+    jp = sub.add_parser("jsense", help="Estimate mps.hdr/.cfl in a separate work directory.")
+    jp.add_argument("input", type=Path)
+    jp.add_argument("--work-dir", type=Path, required=True)
+    jp.add_argument("--device", type=int, default=0, help="GPU device index (the HIP path has no CPU mode).")
+    jp.add_argument("--readout-fraction", type=float, default=0.95,
+                    help="Fraction of readout samples used by JSENSE.")
+    jp.add_argument("--coordinate-scaling", choices=("legacy_max", "physical"), default="legacy_max",
+                    help="Coordinate scaling for JSENSE (legacy_max: the historical per-axis maximum).")
+    jp.add_argument("--fov-scale", type=float, nargs=3, default=(1.0, 1.0, 1.0), metavar=("Z", "Y", "X"),
+                    help="Reconstruction FOV scale in z, y, x order.")
+    jp.add_argument("--show-progress", action="store_true", help="Show a progress bar over the JSENSE rounds.")
     args = p.parse_args(argv)
     logging.basicConfig(level=logging.DEBUG if args.verbose else logging.INFO,
                         format="%(levelname)s: %(message)s")
@@ -283,6 +295,15 @@ def cli_main(argv=None) -> int:
         from .preprocessing import estimate_respiratory_from_dataset
         args.work_dir.mkdir(parents=True, exist_ok=True)
         out = estimate_respiratory_from_dataset(args.input, args.work_dir)
+        print(f"Wrote {out}.hdr and {out}.cfl")
+        return 0
+    if args.command == "jsense":
+        from .jsense import JsenseConfig, estimate_sensitivity_from_dataset
+        args.work_dir.mkdir(parents=True, exist_ok=True)
+        cfg = JsenseConfig(readout_fraction=args.readout_fraction, coordinate_scaling=args.coordinate_scaling)
+        out = estimate_sensitivity_from_dataset(args.input, args.work_dir, device=args.device,
+                                                fov_scale_zyx=args.fov_scale, show_progress=args.show_progress,
+                                                config=cfg)
         print(f"Wrote {out}.hdr and {out}.cfl")
         return 0
     fwd = list(args.algorithm_arguments)

@@ -259,3 +259,13 @@ def test_prepare_trajectory_known_answers():
                            fov_scale_zyx=(0.5, 1, 1))
     with pytest.raises(ValueError, match="readout_fraction"):
         prepare_trajectory(raw, dens, image_shape_zyx=(2, 2, 2), voxel_size_zyx=(1, 1, 1), readout_fraction=0.0)
+
+
+def test_cli_jsense_arguments_and_missing_dataset(tmp_path):
+    from toporecon_hip.mains import cli_main
+    with pytest.raises(FileNotFoundError, match="missing required files"):
+        cli_main(["jsense", str(tmp_path / "nothing"), "--work-dir", str(tmp_path / "w"), "--device", "0",
+                  "--readout-fraction", "0.9", "--coordinate-scaling", "physical", "--fov-scale", "1", "1.5", "1",
+                  "--show-progress"])
+    with pytest.raises(SystemExit):
+        cli_main(["jsense", str(tmp_path), "--work-dir", str(tmp_path), "--coordinate-scaling", "other"])
