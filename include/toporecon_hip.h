@@ -104,7 +104,9 @@ int tr_plan_release_workspace(tr_plan* plan);
 /* TR_PARAM_SPREAD_ORDER (default 65535): order in which the spreading visits grid clusters.
  * 65535: the light clusters holding a support point, longest pair list first, after the heavy
  * clusters' segments; bz | by << 4 | bx << 8 (each >= 1): the same clusters in bricks of
- * bz x by x bx clusters; 0: every cluster in index order.  Results are identical for every order. */
+ * bz x by x bx clusters; 65536 + (bz | by << 4 | bx << 8): those bricks, the brick holding the
+ * longest list first, index order inside a brick; 0: every cluster in index order.  Results are
+ * identical for every order. */
 #define TR_PARAM_SPREAD_ORDER 10
 int tr_plan_set_param(tr_plan* plan, int param, int64_t value);
 int tr_plan_get_param(const tr_plan* plan, int param, int64_t* value);

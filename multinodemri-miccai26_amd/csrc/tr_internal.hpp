@@ -28,6 +28,7 @@ constexpr int kMaxKc = 16;  // channels per FFT workgroup
 #endif
 constexpr int kSpreadSeg = TR_SPREAD_SEG;  // pairs per heavy-cluster segment
 constexpr int kSpreadOrderLongestFirst = 0xffff;  // TR_PARAM_SPREAD_ORDER code of the default order
+constexpr int kSpreadOrderBrickLongestFirst = 0x10000;  // + brick code: bricks, longest list's brick first
 #ifndef TR_SPREAD_UNROLL
 #define TR_SPREAD_UNROLL 4
 #endif
@@ -395,6 +396,14 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s);
 // Oversampled-grid lengths with a compile-time FFT plan for channel chunks of
 // 8 and 16; other (length, chunk) pairs run the runtime-radix kernels.
 #define TR_FAST_LENGTHS(X) X(40) X(50) X(60) X(64) X(80) X(96) X(100) X(120) X(128) X(160) X(200) X(240)
+
+inline bool fast_length(int L) {
+#define TR_IS_FAST_(n) \
+  if (L == n) return true;
+  TR_FAST_LENGTHS(TR_IS_FAST_)
+#undef TR_IS_FAST_
+  return false;
+}
 
 template <class Launch, int KC, class Args>
 inline void launch_len(int L, dim3 g, size_t sm, hipStream_t s, const Args& a) {

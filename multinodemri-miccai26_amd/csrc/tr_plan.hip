@@ -591,8 +591,10 @@ int build_cluster_order(tr_plan* p, BinData& b, int code) {
   b.corder = nullptr;
   b.ncorder = 0;
   const bool lpt = code == kSpreadOrderLongestFirst;
+  const bool brick_lpt = (code & kSpreadOrderBrickLongestFirst) != 0;
   const int bz = lpt ? 1 : std::max(1, code & 15), by = lpt ? 1 : std::max(1, (code >> 4) & 15),
             bx = lpt ? 1 : std::max(1, (code >> 8) & 255);
+  std::vector<int64_t> brick_end;  // end of each brick's run in `order`
   std::vector<int> rp(p->NCL + 1);
   TR_HIP(hipMemcpy(rp.data(), b.row_ptr, sizeof(int) * (p->NCL + 1), hipMemcpyDeviceToHost));
   const int ncz = p->ncl[0], ncy = p->ncl[1], ncx = p->ncl[2];
@@ -602,7 +604,7 @@ int build_cluster_order(tr_plan* p, BinData& b, int code) {
   order.reserve((size_t)p->NCL);
   for (int z0 = 0; z0 < ncz; z0 += bz)
     for (int y0 = 0; y0 < ncy; y0 += by)
-      for (int x0 = 0; x0 < ncx; x0 += bx)
+      for (int x0 = 0; x0 < ncx; x0 += bx) {
         for (int cz = z0; cz < std::min(z0 + bz, ncz); ++cz)
           for (int cy = y0; cy < std::min(y0 + by, ncy); ++cy)
             for (int cx = x0; cx < std::min(x0 + bx, ncx); ++cx) {
@@ -625,9 +627,33 @@ int build_cluster_order(tr_plan* p, BinData& b, int code) {
                 }
               if (live) order.push_back((int)t);
             }
+          if (brick_lpt && (brick_end.empty() ? !order.empty() : brick_end.back() < (int64_t)order.size()))
+            brick_end.push_back((int64_t)order.size());
+        }
   if (code == kSpreadOrderLongestFirst)  // longest pair lists first (index order among equal lengths)
     std::stable_sort(order.begin(), order.end(),
                      [&rp](int x, int y) { return rp[x + 1] - rp[x] > rp[y + 1] - rp[y]; });
+  if (brick_lpt) {
+    // whole bricks, the brick holding the longest list first (brick order among equal maxima);
+    // inside a brick, index order, so clusters that share samples run close together
+    std::vector<std::pair<int, int64_t>> key;  // (longest list, brick index)
+    int64_t s0 = 0;
+    for (size_t i = 0; i < brick_end.size(); ++i) {
+      int m = 0;
+      for (int64_t j = s0; j < brick_end[i]; ++j) m = std::max(m, rp[order[j] + 1] - rp[order[j]]);
+      key.push_back({m, (int64_t)i});
+      s0 = brick_end[i];
+    }
+    std::stable_sort(key.begin(), key.end(), [](const std::pair<int, int64_t>& a,
+                                                const std::pair<int, int64_t>& b) { return a.first > b.first; });
+    std::vector<int> sorted;
+    sorted.reserve(order.size());
+    for (const auto& k : key) {
+      const int64_t i = k.second;
+      for (int64_t j = i == 0 ? 0 : brick_end[i - 1]; j < brick_end[i]; ++j) sorted.push_back(order[j]);
+    }
+    order.swap(sorted);
+  }
   b.ncorder = (int64_t)order.size();
   if (!order.empty()) {
     TR_HIP(hipMalloc(&b.corder, sizeof(int) * order.size()));
@@ -888,7 +914,9 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
       p->lines_xcd_lines = (int)value;
       return kOk;
     case TR_PARAM_SPREAD_ORDER: {
-      if (value < 0 || value > 0xffff) TR_FAIL(kInvalidArgument, "spreading order code outside [0, 65535]");
+      if (value < 0 || value > (kSpreadOrderBrickLongestFirst | 0xffff) ||
+          (value > 0xffff && (value & 0xffff) == 0xffff))
+        TR_FAIL(kInvalidArgument, "spreading order code outside [0, 65535] and 65536 + [1, 65534]");
       TR_HIP(hipDeviceSynchronize());
       for (BinData& b : p->bins) {
         if (value == 0 || b.M == 0) {

@@ -92,6 +92,7 @@ def tune(plan, *, image, maps, E: int, C: int, ksp, dcf, stream, bins=(0,), reps
     best_kg, best_i = min(timings["interp"], key=timings["interp"].get)
     plan.set_param("interp_kgroups", best_kg)
     plan.set_param("interp_xcd_chunk", best_i)
+    adj()  # warm the adjoint path, so the first candidate is not charged for it
     for c in SPREAD_CANDIDATES:
         plan.set_param("spread_xcd_chunk", c)
         timings["spread"][c] = _time_kernel("spread", adj, reps)

@@ -252,8 +252,8 @@ def test_grouped_interp_never_reads_unwritten_workspace(torch_mod):
 
 
 def test_spreading_orders_and_heavy_segments(torch_mod):
-    """The spreading's cluster visiting orders (longest first, bricks, index order over every
-    cluster) and support pruning give identical bits, also for clusters whose pair lists are long
+    """The spreading's cluster visiting orders (longest first, bricks, bricks longest first, index
+    order over every cluster) and support pruning give identical bits, also for clusters whose pair lists are long
     enough to be summed in segments (every cone interleave starts at k = 0, so the centre cluster
     collects thousands of pairs), and the result matches the oracle."""
     torch = torch_mod
@@ -276,7 +276,7 @@ def test_spreading_orders_and_heavy_segments(torch_mod):
     dt = torch.from_numpy(dcf).to(dev)
     s = _lib.stream_handle(dev)
     outs = []
-    for order, prune in ((0xffff, 1), (0x111, 1), (0x822, 1), (0, 1), (0xffff, 0)):
+    for order, prune in ((0xffff, 1), (0x111, 1), (0x822, 1), (0x10422, 1), (0, 1), (0xffff, 0)):
         plan.set_param("spread_order", order)
         plan.set_param("prune", prune)
         g = torch.empty((E, *shape), dtype=torch.complex64, device=dev)
@@ -290,7 +290,9 @@ def test_spreading_orders_and_heavy_segments(torch_mod):
         ref = sum(np.conj(mp[c]) * on.nufft_adjoint(q[e * C + c] * dcf, coord, shape) for c in range(C))
         assert _rel(outs[0][e], ref) < TOL
     with pytest.raises(Exception):
-        plan.set_param("spread_order", 70000)
+        plan.set_param("spread_order", 0x20000)
+    with pytest.raises(Exception):
+        plan.set_param("spread_order", 0x1ffff)
     plan.close()
 
 
