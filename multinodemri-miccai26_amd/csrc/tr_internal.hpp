@@ -397,14 +397,6 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s);
 // 8 and 16; other (length, chunk) pairs run the runtime-radix kernels.
 #define TR_FAST_LENGTHS(X) X(40) X(50) X(60) X(64) X(80) X(96) X(100) X(120) X(128) X(160) X(200) X(240)
 
-inline bool fast_length(int L) {
-#define TR_IS_FAST_(n) \
-  if (L == n) return true;
-  TR_FAST_LENGTHS(TR_IS_FAST_)
-#undef TR_IS_FAST_
-  return false;
-}
-
 template <class Launch, int KC, class Args>
 inline void launch_len(int L, dim3 g, size_t sm, hipStream_t s, const Args& a) {
   switch (L) {

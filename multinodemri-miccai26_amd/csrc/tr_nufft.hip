@@ -185,21 +185,11 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     const int kcc = fft_kc(C, p->ax[2].M);
     dim3 grid((unsigned)(nz * ny));
     ProfScope ps(kKAdjXCrop, 8.0 * K * nz * ny * gx + 8.0 * C * p->N + 8.0 * E * p->N * (accumulate ? 2 : 1), s);
-    // the two-level kernels keep their (echo, column) sums in LDS after the tile: as many echoes
-    // per launch as fit 64 KiB (at most kAdjXEchoes)
-    const size_t tile = fft_smem(p->ax[2].M, kcc);
-    const bool two_level = kcc == 16 && fast_length(gx) && split_n1(gx) > 0;
-    const size_t col = two_level ? sizeof(float2) * (size_t)p->n[2] : 0;
-    int eg = kAdjXEchoes;
-    if (two_level) {
-      if (tile + col > 65536) TR_FAIL(kUnsupported, "image rows too long for the inverse x pass (LDS)");
-      eg = (int)std::min<size_t>((size_t)kAdjXEchoes, (65536 - tile) / col);
-    }
-    for (int e0 = 0; e0 < E; e0 += eg) {
-      a.E = std::min(eg, E - e0);
+    for (int e0 = 0; e0 < E; e0 += kAdjXEchoes) {  // at most kAdjXEchoes per launch
+      a.E = std::min(kAdjXEchoes, E - e0);
       a.grid = p->work + (int64_t)e0 * C;
       a.out = out + (int64_t)e0 * p->N;
-      if (int r = launch_adj_x(kcc, gx, grid, tile + col * (size_t)a.E, s, a)) return r;
+      if (int r = launch_adj_x(kcc, gx, grid, fft_smem(p->ax[2].M, kcc), s, a)) return r;
     }
   }
   return kOk;

@@ -160,22 +160,29 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
   if constexpr (two_level_ok<KC, LC>()) {
     // two-level register FFT; pass 2 multiplies its outputs by conj(map) and writes them to
     // LDS by image column, where thread x adds the chunk's coils in increasing order (the same
-    // products in the same order as the stage-by-stage path).  The per-(echo, column) sums live
-    // in LDS after the tile (thread x only ever touches column x), so no echo's state sits in
-    // registers and the echo loop stays rolled: 238 -> far fewer VGPRs, more waves per SIMD.
+    // products in the same order as the stage-by-stage path)
     constexpr int N1 = split_n1(LC), N2 = LC / N1;
-    float2* accs = buf + (size_t)LF * T::ld;  // [e][x], E x nx
-    if (tid < nx)
-      for (int e = 0; e < E; ++e) accs[e * nx + tid] = make_float2(0.f, 0.f);
+    float2 acc[kMaxE];
+#pragma unroll
+    for (int e = 0; e < kMaxE; ++e) acc[e] = make_float2(0.f, 0.f);
     for (int c0 = 0; c0 < a.C; c0 += KC) {
       const int cn = min(KC, a.C - c0);
       // clamped, unconditional loads (see the forward pass); results are selected afterwards
       const int chc = c0 + min(ch, cn - 1);
-      // thread x multiplies column x of the chunk's coils by conj(map) while it adds them up
-      const float2* mcol = (a.maps && tid < nx) ? a.maps + (int64_t)c0 * N + rowvox + tid : nullptr;
-#pragma unroll 1
-      for (int e = 0; e < E; ++e) {
-        {
+      float2 mp[N2];
+      if (a.maps) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+          mp[k2] = a.maps[(int64_t)chc * N + rowvox + min(pmod(w + N1 * k2 + nx / 2, gx), nx - 1)];
+      } else {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) mp[k2] = make_float2(1.f, 0.f);
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) keep_loaded(mp[k2]);
+#pragma unroll
+      for (int e = 0; e < kMaxE; ++e) {
+        if (e < E) {
           float2 v[N1];
           const bool lv = w < N2 && ch < cn;
 #pragma unroll
@@ -195,29 +202,27 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
             for (int k2 = 0; k2 < N2; ++k2) {
               const int x = pmod(w + N1 * k2 + nx / 2, gx);
               if (x < nx) {
-                buf[x * T::ld + ch] = cconj(zz[k2]);
+                const float2 r = cconj(zz[k2]);
+                buf[x * T::ld + ch] = a.maps ? cmulc(mp[k2], r) : r;
               }
             }
           }
           __syncthreads();
-          if (tid < nx) {
-            float2 acc = accs[e * nx + tid];
-            for (int cc = 0; cc < cn; ++cc) {
-              const float2 r = buf[tid * T::ld + cc];
-              acc = cadd(acc, mcol ? cmulc(mcol[(int64_t)cc * N], r) : r);  // the row's maps stay in cache
-            }
-            accs[e * nx + tid] = acc;
-          }
+          if (tid < nx)
+            for (int cc = 0; cc < cn; ++cc) acc[e] = cadd(acc[e], buf[tid * T::ld + cc]);
         }
       }
     }
     if (tid < nx) {
       const int x = tid;
       const float sc = (a.apod_z[z] * a.apod_y[y] * a.apod_x[x]) * a.scale;
-      for (int e = 0; e < E; ++e) {
-        const float2 r = cscale(accs[e * nx + x], sc);
-        float2* o = a.out + (int64_t)e * N + rowvox + x;
-        *o = a.accumulate ? cadd(*o, r) : r;
+#pragma unroll
+      for (int e = 0; e < kMaxE; ++e) {
+        if (e < E) {
+          const float2 r = cscale(acc[e], sc);
+          float2* o = a.out + (int64_t)e * N + rowvox + x;
+          *o = a.accumulate ? cadd(*o, r) : r;
+        }
       }
     }
     return;
