@@ -254,3 +254,30 @@ def test_operator_norm_power_iteration(torch_mod, alg):
     print(alg, "K^2 gpu", est["K_norm_sq"], "oracle", lam, "sigma tau K^2", est["sigma_tau_K_norm_sq"], flush=True)
     assert abs(est["K_norm_sq"] - lam) / lam < 1e-2
     assert est["sigma_tau_K_norm_sq"] == pytest.approx(lam / 36.0, rel=1e-2)
+
+
+def test_steps_beyond_the_bound_run_to_the_cap_like_the_oracle(torch_mod):
+    """sigma = tau = 1 on the TVME test problem puts sigma * tau * ||K||^2 far above PDHG's bound of 1
+    (the reference fixes sigma = tau = 1/6 and, at full size, lands at 1.28): the relative change stops
+    falling and rises again, the solve runs to max_iter without an error and with one history entry
+    per iteration (the reference's behaviour: its overflow back-off never triggers), and the iterates
+    and stop-rule values follow the float64 oracle's."""
+    from toporecon_hip.algorithms import TvmeReconstructor
+    print("steps beyond the bound", flush=True)
+    acq, ksp, prob = make_problem(seed=3, algorithm="tvme", l2=True)
+    B = len(acq.bin_edges) - 1
+    s = TvmeReconstructor(ksp, acq.coord, acq.dcf, acq.mps, acq.resp, np.zeros_like(ksp), B, True, lamda_m=2e-3,
+                          lamda_e=2e-3, sigma=1.0, tau=1.0, max_iter=12, tol=1e-9, margin=0, device=0,
+                          bin_edges=acq.bin_edges, show_pbar=False)
+    kn = s.operator_norm(iters=20)
+    assert kn["sigma_tau_K_norm_sq"] > 4.0, kn
+    ubar = s.run().cpu().numpy()
+    assert s.iterations == 12 and len(s.history) == 12
+    hist = []
+    o_ubar, n, _ = op.run(prob, "tvme", lam_m=2e-3, lam_e=2e-3, sigma=1.0, tau=1.0, l2_coupling=True, max_iter=12,
+                          tol=1e-9, history=hist)
+    assert n == 12
+    assert min(hist) < hist[-1] and min(s.history) < s.history[-1]  # not settling
+    assert np.all(np.isfinite(ubar))
+    np.testing.assert_allclose(s.history, hist, rtol=1e-3)
+    assert rel(ubar, o_ubar) < 1e-4
