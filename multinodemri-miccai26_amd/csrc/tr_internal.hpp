@@ -388,6 +388,9 @@ int check_plan_bin(const tr_plan* p, int bin);
 
 // Kernel launchers (one source file per kernel family).
 int launch_fwd_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const FwdXArgs& a);
+// The inverse x pass without a coil sum (C == 1, no maps): grid (image rows, channel chunks of kc),
+// out[k] = crop(IFFT_x(channel k)) * apodisation * scale for every channel k.
+int launch_adj_x_channels(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a);
 int launch_adj_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a);
 int launch_lines(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const LinesArgs& a);
 int launch_interp(const InterpArgs& a, hipStream_t s);

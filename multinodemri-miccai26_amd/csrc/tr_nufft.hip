@@ -182,9 +182,17 @@ static int grid_to_image(tr_plan* p, const float2* maps, int E, int C, float2* o
     a.C = C;
     a.K = K;
     a.accumulate = accumulate;
+    ProfScope ps(kKAdjXCrop, 8.0 * K * nz * ny * gx + 8.0 * C * p->N + 8.0 * E * p->N * (accumulate ? 2 : 1), s);
+    if (!maps && C == 1) {  // no coil sum: every channel is its own image, FFTs run many channels wide
+      const int kc = fft_kc(K, p->ax[2].M);
+      a.E = K;
+      a.grid = p->work;
+      a.out = out;
+      dim3 grid((unsigned)(nz * ny), (unsigned)((K + kc - 1) / kc));
+      return launch_adj_x_channels(kc, gx, grid, fft_smem(p->ax[2].M, kc), s, a);
+    }
     const int kcc = fft_kc(C, p->ax[2].M);
     dim3 grid((unsigned)(nz * ny));
-    ProfScope ps(kKAdjXCrop, 8.0 * K * nz * ny * gx + 8.0 * C * p->N + 8.0 * E * p->N * (accumulate ? 2 : 1), s);
     for (int e0 = 0; e0 < E; e0 += kAdjXEchoes) {  // at most kAdjXEchoes per launch
       a.E = std::min(kAdjXEchoes, E - e0);
       a.grid = p->work + (int64_t)e0 * C;

@@ -275,6 +275,56 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Inverse pass along x without a coil sum (C == 1, no maps): every grid channel k is its own
+// image.  Block = (image row, chunk of KC channels), so the line FFT runs KC channels wide rather
+// than one image at a time; the crop, apodisation and scale are those of k_adj_x_crop (the same
+// operations in the same order: 0 + conj(X), times (az ay ax) scale).
+// ---------------------------------------------------------------------------
+template <int KC, int LC>
+__global__ void __launch_bounds__(kBlock) k_adj_x_channels(AdjXArgs a) {
+  extern __shared__ float2 smem[];
+  using T = Tile<KC, LC>;
+  const int LF = LC > 0 ? LC : a.ax.M;
+  float2* tw = smem;
+  float2* buf = smem + LF;
+  const int tid = threadIdx.x;
+  const int ch = tid % KC, w = tid / KC;
+  const int nz = a.gg.n[0], ny = a.gg.n[1], nx = a.gg.n[2];
+  const int gz = a.gg.g[0], gy = a.gg.g[1], gx = a.gg.g[2];
+  const int row = blockIdx.x;
+  const int k0 = blockIdx.y * KC;
+  const int cn = min(KC, a.K - k0);
+  const int z = row / ny, y = row - (row / ny) * ny;
+  const int64_t N = (int64_t)nz * ny * nx;
+  for (int i = tid; i < LF; i += kBlock) tw[i] = a.tw[i];
+  const int zp = pmod(z - nz / 2, gz), yp = pmod(y - ny / 2, gy);
+  const float2* src = a.grid + (((int64_t)zp * gy + yp) * gx) * a.K + k0;
+  const int64_t rowvox = ((int64_t)z * ny + y) * nx;
+  float2 v[T::PER];
+#pragma unroll
+  for (int it = 0; it < T::PER; ++it) {
+    const int p = w + it * T::W;
+    v[it] = (p < gx && ch < cn) ? src[(int64_t)p * a.K + ch] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int it = 0; it < T::PER; ++it) {
+    const int p = w + it * T::W;
+    if (p < gx) buf[p * T::ld + ch] = cconj(v[it]);
+  }
+  __syncthreads();
+  fft_any<T::W, LC>(buf, T::ld, ch, w, a.ax, tw);
+  __syncthreads();
+  for (int i = tid; i < cn * nx; i += kBlock) {
+    const int cc = i / nx, x = i - cc * nx;
+    const int p = pmod(x - nx / 2, gx);
+    const float sc = (a.apod_z[z] * a.apod_y[y] * a.apod_x[x]) * a.scale;
+    const float2 r = cscale(cadd(make_float2(0.f, 0.f), cconj(buf[p * T::ld + cc])), sc);
+    float2* o = a.out + (int64_t)(k0 + cc) * N + rowvox + x;
+    *o = a.accumulate ? cadd(*o, r) : r;
+  }
+}
+
 struct FwdXLaunch {
   template <int KC, int LC>
   static void go(dim3 g, size_t sm, hipStream_t s, const FwdXArgs& a) {
@@ -291,6 +341,17 @@ struct AdjXLaunch {
 
 int launch_fwd_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const FwdXArgs& a) {
   return launch_fft_kernel<FwdXLaunch>(kc, L, grid, smem, s, a);
+}
+
+struct AdjXChannelsLaunch {
+  template <int KC, int LC>
+  static void go(dim3 g, size_t sm, hipStream_t s, const AdjXArgs& a) {
+    hipLaunchKernelGGL((k_adj_x_channels<KC, LC>), g, dim3(kBlock), sm, s, a);
+  }
+};
+
+int launch_adj_x_channels(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a) {
+  return launch_fft_kernel<AdjXChannelsLaunch>(kc, L, grid, smem, s, a);
 }
 
 int launch_adj_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a) {
