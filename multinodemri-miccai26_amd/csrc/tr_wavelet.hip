@@ -49,16 +49,20 @@ __global__ void k_dwt_axis(AxisArgs a, const float2* __restrict__ x, float2* __r
   const int64_t img = row / a.rows_img;
   const int64_t r = row - img * a.rows_img;
   const float2* xs = x + img * a.sig_stride + r * a.n * a.Q + q;
+  // every tap is loaded from a clamped, in-bounds index before any is used (no load sits behind a
+  // per-lane branch); taps outside [0, n) are then skipped as before
+  float2 v[kFilt];
+#pragma unroll
+  for (int j = 0; j < kFilt; ++j) v[j] = xs[(int64_t)min(max(2 * k + 1 - j, 0), a.n - 1) * a.Q];
   float2 sl = make_float2(0.f, 0.f), sh = sl;
 #pragma unroll
   for (int j = 0; j < kFilt; ++j) {
     const int m = 2 * k + 1 - j;
     if (m >= 0 && m < a.n) {
-      const float2 v = xs[(int64_t)m * a.Q];
-      sl.x += c_dec_lo[j] * v.x;
-      sl.y += c_dec_lo[j] * v.y;
-      sh.x += c_dec_hi[j] * v.x;
-      sh.y += c_dec_hi[j] * v.y;
+      sl.x += c_dec_lo[j] * v[j].x;
+      sl.y += c_dec_lo[j] * v[j].y;
+      sh.x += c_dec_hi[j] * v[j].x;
+      sh.y += c_dec_hi[j] * v[j].y;
     }
   }
   const int64_t o = (r * a.c + k) * a.Q + q;
@@ -80,15 +84,27 @@ __global__ void k_idwt_axis(AxisArgs a, const float2* __restrict__ lo, const flo
   const int64_t r = row - img * a.rows_img;
   const float2* ls = lo + img * a.lo_stride + a.lo_off + r * a.c * a.Q + q;
   const float2* hs = hi + img * a.hi_stride + a.hi_off + r * a.c * a.Q + q;
-  // taps j = 2k + 1 - m in [0, 11]  <=>  m / 2 <= k <= (m + 10) / 2
-  const int k1 = min(a.c - 1, (m + 10) / 2);
+  // taps j = 2k + 1 - m in [0, 11]  <=>  k = m / 2 + i, i = 0 .. 5 (j = 2i + 1 for even m, 2i for
+  // odd m), and k <= c - 1.  All 12 values are loaded (clamped, in bounds) before any is used; the
+  // sum runs over increasing k as before.
+  const int kb = m / 2;
+  const bool odd = (m & 1) != 0;
+  float2 vl[6], vh[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const int64_t kc = min(kb + i, a.c - 1);
+    vl[i] = ls[kc * a.Q];
+    vh[i] = hs[kc * a.Q];
+  }
   float2 s = make_float2(0.f, 0.f);
-  for (int k = m / 2; k <= k1; ++k) {
-    const int j = 2 * k + 1 - m;
-    const float2 vl = ls[(int64_t)k * a.Q];
-    const float2 vh = hs[(int64_t)k * a.Q];
-    s.x += c_dec_lo[j] * vl.x + c_dec_hi[j] * vh.x;
-    s.y += c_dec_lo[j] * vl.y + c_dec_hi[j] * vh.y;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if (kb + i <= a.c - 1) {
+      const float fl = odd ? c_dec_lo[2 * i] : c_dec_lo[2 * i + 1];
+      const float fh = odd ? c_dec_hi[2 * i] : c_dec_hi[2 * i + 1];
+      s.x += fl * vl[i].x + fh * vh[i].x;
+      s.y += fl * vl[i].y + fh * vh[i].y;
+    }
   }
   x[img * a.sig_stride + (r * a.n + m) * a.Q + q] = s;
 }
