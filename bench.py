@@ -1040,9 +1040,9 @@ def bench_full(args, verbose=True):
         log("no trained warm-start weights: the learned leg is skipped (plain PDHG only)", verbose)
 
     def learned_solve(progress=0):
-        u0 = warm_start(solver, net, cg_steps=cg_steps)
-        u = solver.run(init=u0, dual_init="residual", progress_every=progress)
-        del u0
+        u0, q0 = warm_start(solver, net, cg_steps=cg_steps, with_dual=True)
+        u = solver.run(init=u0, dual_init=q0, progress_every=progress)
+        del u0, q0
         return u
 
     # warmup: learned solves capped at WARMUP_ITERS PDHG iterations (every kernel and code path runs)
@@ -1126,12 +1126,12 @@ def bench_full(args, verbose=True):
         kind, k_cg = v.split("+")
         torch.cuda.synchronize(dev)
         ta = time.perf_counter()
-        u0 = warm_start(solver, net if kind == "cnn" else None, cg_steps=int(k_cg))
-        u_v = solver.run(init=u0, dual_init="residual")
+        u0, q0 = warm_start(solver, net if kind == "cnn" else None, cg_steps=int(k_cg), with_dual=True)
+        u_v = solver.run(init=u0, dual_init=q0)
         torch.cuda.synchronize(dev)
         tv = time.perf_counter() - ta
         phi = objective.objective(solver, u_v)
-        del u0, u_v
+        del u0, q0, u_v
         variants.append({"start": f"{kind}+{k_cg}cg", "time_to_tol_s": tv, "iterations": solver.iterations,
                          "objective_rel_diff": (phi - phi_plain) / phi_plain, "speedup_vs_plain": t_plain / tv})
         log(f"variant {v}: {solver.iterations} iterations, {tv:.2f} s, objective "

@@ -258,6 +258,10 @@ int tr_cnn_sumsq(const tr_c64* img, int64_t V, int n, double* out, void* stream)
  * work holds 2 * nblocks * K doubles): the least-squares scale of a predicted image. */
 int tr_data_fit_sums(const tr_c64* a_sorted, const tr_c64* y_sorted, const float* dcf_sorted, int64_t M, int K,
                      double* work, int nblocks, double* out, void* stream);
+/* q[m, k] *= scale[k] on sorted samples (M, K), scale a device array of K floats: A (c v) from A v
+ * for a per-(bin, echo) scale c, so the warm start's CG needs no second forward NUFFT (no reference
+ * counterpart). */
+int tr_data_scale_channels(tr_c64* q_sorted, int64_t M, int K, const float* scale, void* stream);
 
 /* ---- JSENSE coil-sensitivity estimation ----
  * Replace the linear operators of SigPy 0.1.27's sigpy.mri.app.JsenseRecon, which the reference

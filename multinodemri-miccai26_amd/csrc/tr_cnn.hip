@@ -362,6 +362,18 @@ __global__ void k_fit_reduce(const double2* partial, int nblocks, int K, double2
   out[k] = make_double2(s1, s2);
 }
 
+// q[m, k] *= scale[k] over sorted samples (M, K): per-channel scaling of k-space.
+__global__ void k_scale_channels(float2* __restrict__ q, int64_t n, int K, const float* __restrict__ scale) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float c = scale[i % K];
+    float2 v = q[i];
+    v.x *= c;
+    v.y *= c;
+    q[i] = v;
+  }
+}
+
 }  // namespace tr
 
 using namespace tr;
@@ -472,6 +484,18 @@ int tr_cnn_sumsq(const tr_c64* img, int64_t V, int n, double* out, void* stream)
   if (!img || !out) TR_FAIL(kInvalidArgument, "null argument");
   if (n < 1) return kOk;
   hipLaunchKernelGGL(k_cnn_sumsq, dim3(n), dim3(256), 0, (hipStream_t)stream, F2(img), V, out);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_data_scale_channels(tr_c64* q_sorted, int64_t M, int K, const float* scale, void* stream) {
+  if (M < 0 || K < 1) TR_FAIL(kInvalidArgument, "need M >= 0 and K >= 1");
+  if (M == 0) return kOk;
+  if (!q_sorted || !scale) TR_FAIL(kInvalidArgument, "null argument");
+  const int64_t n = M * K;
+  const int64_t b = (n + 255) / 256;
+  hipLaunchKernelGGL(k_scale_channels, dim3((unsigned)(b > 65536 ? 65536 : b)), dim3(256), 0, (hipStream_t)stream,
+                     F2(q_sorted), n, K, scale);
   TR_LAUNCH_CHECK();
   return kOk;
 }

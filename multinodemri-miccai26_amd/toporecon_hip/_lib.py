@@ -79,6 +79,7 @@ SIGNATURES = {
     "tr_cnn_unpack": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp]),
     "tr_cnn_sumsq": (c_int, [c_vp, c_i64, c_int, c_vp, c_vp]),
     "tr_data_fit_sums": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp]),
+    "tr_data_scale_channels": (c_int, [c_vp, c_i64, c_int, c_vp, c_vp]),
     "tr_js_work_count": (c_i64, [c_int, c_int, c_int, c_int, c_int]),
     "tr_js_conv": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
     "tr_js_conv_adj_img": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),

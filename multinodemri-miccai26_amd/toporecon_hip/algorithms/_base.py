@@ -384,18 +384,25 @@ class HipPdhgBase:
 
         init: starting image (B, E, Z, Y, X) instead of zeros (the reference
         always starts from zeros).  dual_init="residual" with an init also
-        starts the data dual at its fixed point A init - y; otherwise the data
-        dual starts at the constructor's dual_q (or zero).  progress_every > 0
-        prints the iteration and stopping value to stderr every that many
-        iterations.
+        starts the data dual at its fixed point A init - y; a sequence of B
+        arrays ((M_b, E*C) sorted, e.g. from warm_start(..., with_dual=True))
+        starts it there; otherwise the data dual starts at the constructor's
+        dual_q (or zero).  progress_every > 0 prints the iteration and
+        stopping value to stderr every that many iterations.
         """
         import sys
         import torch
-        if dual_init not in (None, "residual"):
+        given = isinstance(dual_init, (list, tuple))
+        if not given and dual_init not in (None, "residual"):
             raise ValueError(f"unknown dual_init {dual_init!r}")
+        if given and len(dual_init) != self.B:
+            raise ValueError(f"dual_init has {len(dual_init)} bins, the solver {self.B}")
         with torch.cuda.device(self.device):
             self.pdhg_init(init)
-            if init is not None and dual_init == "residual":
+            if given:
+                for b in range(self.B):
+                    self.q[b].copy_(dual_init[b])
+            elif init is not None and dual_init == "residual":
                 self.init_dual_residual(self.ubar)
             else:
                 self.reset_dual()

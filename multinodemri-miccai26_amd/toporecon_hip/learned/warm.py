@@ -75,19 +75,23 @@ def predict(net, img, shape, chunk=6):
     return v, x, out, scale
 
 
-def fit_scales(solver, v):
-    """Least-squares scale per (bin, echo) of v (B, E, Z, Y, X) against the data (float64)."""
+def fit_scales(solver, v, keep_av=False):
+    """Least-squares scale per (bin, echo) of v (B, E, Z, Y, X) against the data (float64); with
+    keep_av, also A v per bin ((M_b, E*C) sorted k-space), which the caller may reuse."""
     import torch
     s = solver.stream
     K = solver.E * solver.C
     c = np.zeros((solver.B, solver.E))
     work = torch.empty((FIT_BLOCKS, K, 2), dtype=torch.float64, device=solver.device)
     sums = torch.empty((K, 2), dtype=torch.float64, device=solver.device)
+    avs = []
     for b in range(solver.B):
         M = solver.y[b].shape[0]
+        av = torch.empty_like(solver.y[b])
+        if keep_av:
+            avs.append(av)
         if M == 0:
             continue
-        av = torch.empty_like(solver.y[b])
         _lib.call("tr_data_simulate", solver.plan.handle, b, v[b].data_ptr(), solver.maps.data_ptr(), solver.E,
                   solver.C, av.data_ptr(), s)
         _lib.call("tr_data_fit_sums", av.data_ptr(), solver.y[b].data_ptr(), solver.dcf[b].data_ptr(), M, K,
@@ -95,27 +99,36 @@ def fit_scales(solver, v):
         h = sums.cpu().numpy().reshape(solver.E, solver.C, 2).sum(axis=1)
         c[b] = np.where(h[:, 1] > 0, h[:, 0] / np.where(h[:, 1] > 0, h[:, 1], 1.0), 0.0)
         del av
-    return c
+    return (c, avs) if keep_av else c
 
 
-def cnn_warm_start(solver, net, chunk=6, keep_rhs=False):
+def cnn_warm_start(solver, net, chunk=6, keep_rhs=False, keep_ax=False):
     """u0 (B, E, Z, Y, X) complex64 for ``solver.run(init=u0, dual_init="residual")``; with
-    keep_rhs, (u0, A^H W y) -- the network's input, which ``cg_refine`` can reuse as its rhs."""
+    keep_rhs, also A^H W y (the network's input, which ``cg_refine`` can reuse as its rhs); with
+    keep_ax, also A u0 per bin (sorted k-space), formed as c * A v from the scale fit (A is linear)
+    instead of another forward NUFFT.  Returns u0, or a tuple (u0[, rhs][, ax])."""
     import torch
     shape = tuple(solver.img_shape)
-    B, E = solver.B, solver.E
+    B, E, C = solver.B, solver.E, solver.C
     V = int(np.prod(shape))
     g = dc_adjoint(solver)
     imgs = g.reshape(B * E, *shape)
     v, x, out, scale = predict(net, imgs, shape, chunk=chunk)
     if not keep_rhs:
         del g
-    c = fit_scales(solver, v.reshape(B, E, *shape))
+    fit = fit_scales(solver, v.reshape(B, E, *shape), keep_av=keep_ax)
+    c, avs = fit if keep_ax else (fit, None)
     scale2 = scale * torch.from_numpy(c.reshape(-1).astype(np.float32)).to(scale.device)
     _lib.call("tr_cnn_unpack", x.data_ptr(), out.data_ptr(), scale2.data_ptr(), V, B * E, v.data_ptr(),
               net.stream)
     u0 = v.reshape(B, E, *shape)
-    return (u0, g) if keep_rhs else u0
+    if keep_ax:
+        for b in range(B):
+            cb = torch.from_numpy(np.repeat(c[b], C).astype(np.float32)).to(solver.device)  # channel e*C + c
+            _lib.call("tr_data_scale_channels", avs[b].data_ptr(), int(avs[b].shape[0]), E * C, cb.data_ptr(),
+                      solver.stream)
+    res = (u0,) + ((g,) if keep_rhs else ()) + ((avs,) if keep_ax else ())
+    return res if len(res) > 1 else u0
 
 
 def _host_sum(partial):
@@ -132,19 +145,35 @@ def _normal(solver, x, out, qtmp):
                   solver.maps.data_ptr(), solver.E, solver.C, out[b].data_ptr(), 0, s)
 
 
-def cg_refine(solver, x0, steps: int, rhs=None, x0_zero: bool = False):
+def _data_adjoint_all(solver, q, out):
+    """out = A^H W q over all bins, q per bin in sorted order."""
+    s = solver.stream
+    for b in range(solver.B):
+        _lib.call("tr_data_adjoint", solver.plan.handle, b, q[b].data_ptr(), solver.dcf[b].data_ptr(),
+                  solver.maps.data_ptr(), solver.E, solver.C, out[b].data_ptr(), 0, s)
+
+
+def cg_refine(solver, x0, steps: int, rhs=None, x0_zero: bool = False, ax0=None, track_ax: bool = False):
     """`steps` conjugate-gradient iterations on A^H W A u = A^H W y from x0 (returns a new array).
 
     rhs: A^H W y if the caller has it (the buffer is consumed as the residual); x0_zero: x0 is
-    known to be zero, so the initial residual is the rhs itself."""
+    known to be zero, so the initial residual is the rhs itself; ax0: A x0 per bin (sorted k-space)
+    if the caller has it, so the initial residual needs only the adjoint.  track_ax: keep A x in
+    k-space along the steps (A x += alpha A p, with the A p each step computes anyway) and return
+    (x, A x per bin) -- ax0 (updated in place) or zeros when x0 is zero."""
     import torch
-    if steps <= 0:
+    if steps <= 0 and not track_ax:
         return x0
     s = solver.stream
     n = x0.numel()
     dev = x0.device
     x = torch.empty_like(x0)
     _lib.call("tr_cscale", x0.data_ptr(), n, 1.0, x.data_ptr(), s)
+    ax = None
+    if track_ax:
+        ax = ax0 if ax0 is not None else [torch.zeros_like(solver.y[b]) for b in range(solver.B)]
+    if steps <= 0:
+        return x, ax
     r = rhs if rhs is not None else dc_adjoint(solver)
     qtmp = [torch.empty_like(solver.y[b]) for b in range(solver.B)]
     ap = torch.empty_like(x)
@@ -153,7 +182,10 @@ def cg_refine(solver, x0, steps: int, rhs=None, x0_zero: bool = False):
     if x0_zero:
         _lib.call("tr_cdot_re", r.data_ptr(), r.data_ptr(), n, partial.data_ptr(), CG_BLOCKS, s)
     else:
-        _normal(solver, x, ap, qtmp)
+        if ax0 is not None:
+            _data_adjoint_all(solver, ax0, ap)
+        else:
+            _normal(solver, x, ap, qtmp)
         # r = A^H W y - A^H W A x (p is scratch for the x-side update here)
         _lib.call("tr_cg_update", p.data_ptr(), r.data_ptr(), ap.data_ptr(), ap.data_ptr(), n, 1.0,
                   partial.data_ptr(), CG_BLOCKS, s)
@@ -170,20 +202,36 @@ def cg_refine(solver, x0, steps: int, rhs=None, x0_zero: bool = False):
         alpha = rr / pap
         _lib.call("tr_cg_update", x.data_ptr(), r.data_ptr(), p.data_ptr(), ap.data_ptr(), n, alpha,
                   partial.data_ptr(), CG_BLOCKS, s)
+        if ax is not None:  # A x += alpha A p (qtmp holds A p of this step)
+            for b in range(solver.B):
+                _lib.call("tr_js_axpy", ax[b].data_ptr(), qtmp[b].data_ptr(), qtmp[b].numel(), alpha, s)
         rr_new = _host_sum(partial)
         _lib.call("tr_cg_direction", p.data_ptr(), r.data_ptr(), n, rr_new / rr, s)
         rr = rr_new
     del qtmp, ap, p, r
-    return x
+    return (x, ax) if track_ax else x
 
 
-def warm_start(solver, net=None, cg_steps: int = DEFAULT_CG_STEPS, chunk=6):
-    """Warm start u0: the network's estimate (zeros without a network), then cg_steps CG steps."""
+def warm_start(solver, net=None, cg_steps: int = DEFAULT_CG_STEPS, chunk=6, with_dual: bool = False):
+    """Warm start u0: the network's estimate (zeros without a network), then cg_steps CG steps.
+
+    with_dual: return (u0, q0) with q0 = A u0 - y per bin (sorted), the data dual's fixed point, for
+    ``solver.run(init=u0, dual_init=q0)``; A u0 comes from the scale fit's A v and the CG steps' A p
+    (A is linear), so no forward NUFFT is spent on it."""
     import torch
     if net is not None:
-        if cg_steps <= 0:
-            return cnn_warm_start(solver, net, chunk=chunk)
-        u0, rhs = cnn_warm_start(solver, net, chunk=chunk, keep_rhs=True)
-        return cg_refine(solver, u0, cg_steps, rhs=rhs)
-    u0 = torch.zeros(solver.vol_dims(), dtype=torch.complex64, device=solver.device)
-    return cg_refine(solver, u0, cg_steps, x0_zero=True)
+        if not with_dual:
+            if cg_steps <= 0:
+                return cnn_warm_start(solver, net, chunk=chunk)
+            u0, rhs, ax0 = cnn_warm_start(solver, net, chunk=chunk, keep_rhs=True, keep_ax=True)
+            return cg_refine(solver, u0, cg_steps, rhs=rhs, ax0=ax0)
+        u0, rhs, ax0 = cnn_warm_start(solver, net, chunk=chunk, keep_rhs=True, keep_ax=True)
+        u, ax = cg_refine(solver, u0, cg_steps, rhs=rhs, ax0=ax0, track_ax=True)
+    else:
+        u0 = torch.zeros(solver.vol_dims(), dtype=torch.complex64, device=solver.device)
+        if not with_dual:
+            return cg_refine(solver, u0, cg_steps, x0_zero=True)
+        u, ax = cg_refine(solver, u0, cg_steps, x0_zero=True, track_ax=True)
+    for b in range(solver.B):  # q0 = A u - y
+        _lib.call("tr_js_axpy", ax[b].data_ptr(), solver.y[b].data_ptr(), ax[b].numel(), -1.0, solver.stream)
+    return u, ax

@@ -217,8 +217,8 @@ def run_main(algorithm: str, argv=None) -> int:
         if not os.path.exists(path):
             raise FileNotFoundError(f"--warm-start learned: no weights at {path}")
         net = WarmStartNet.load(path, args.device)
-        u0 = warm_start(rec, net, cg_steps=_cg_steps(args))
-        img = rec.run(init=u0, dual_init="residual").cpu().numpy()
+        u0, q0 = warm_start(rec, net, cg_steps=_cg_steps(args), with_dual=True)
+        img = rec.run(init=u0, dual_init=q0).cpu().numpy()
     else:
         img = rec.run().cpu().numpy()
     logging.info("Reconstruction done in %.2f seconds (%d iterations).", time.perf_counter() - t0,
