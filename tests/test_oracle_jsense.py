@@ -269,3 +269,20 @@ def test_cli_jsense_arguments_and_missing_dataset(tmp_path):
                   "--show-progress"])
     with pytest.raises(SystemExit):
         cli_main(["jsense", str(tmp_path), "--work-dir", str(tmp_path), "--coordinate-scaling", "other"])
+
+
+def test_prepare_trajectory_reference_known_answers():
+    """The reference's own cases (tests/unit/test_trajectory.py:11-44): physical scaling reverses
+    [x, y, z] once and applies voxel x shape x FOV ((1, 2, 3) -> (30, 10, 3) with the FOV doubling z),
+    readout fraction 0.75 of 4 -> 3, the FOV-scaled shape; a density of the wrong shape is refused."""
+    from toporecon_hip.data.trajectory import prepare_trajectory
+    raw = np.zeros((2, 4, 3), np.float32)
+    raw[..., 0], raw[..., 1], raw[..., 2] = 1.0, 2.0, 3.0
+    r = prepare_trajectory(raw, np.ones((2, 4), np.float32), image_shape_zyx=(10, 20, 30),
+                           voxel_size_zyx=(0.5, 0.25, 0.1), readout_fraction=0.75, fov_scale_zyx=(2.0, 1.0, 1.0),
+                           scaling="physical", unstable_coordinate_threshold=None)
+    assert r.readout == 3 and r.image_shape_zyx == (20, 20, 30)
+    np.testing.assert_allclose(r.coordinates_zyx[0, 0], (30.0, 10.0, 3.0), rtol=1e-6)
+    with pytest.raises(ValueError, match="density shape"):
+        prepare_trajectory(np.zeros((2, 4, 3), np.float32), np.zeros((2, 3), np.float32), image_shape_zyx=(4, 4, 4),
+                           voxel_size_zyx=(1, 1, 1), readout_fraction=1.0)
