@@ -57,6 +57,26 @@ int tr_graph_end(void* stream, tr_graph** out);
 int tr_graph_launch(tr_graph* graph, void* stream);
 int tr_graph_destroy(tr_graph* graph);
 
+/* ---- Gated launches for many instances in one graph (no reference counterpart: the
+ * reference runs one problem per invocation and stops on the host, tvm.py:416-471).
+ * tr_gate_set(gate) makes every solver kernel this host thread launches afterwards (NUFFT
+ * passes, PDHG element-wise kernels, wavelets, reductions) return at once while *gate != 0;
+ * NULL removes the gate.  The gate pointer is read when a kernel is launched (or captured),
+ * its value when the kernel runs.
+ * tr_stop_rule applies the stopping rule on the device to the float64 sums (num, den) of
+ * one iteration: unless *gate is already nonzero it increments *its, stores the stopping
+ * value in hist[*its - 1] and sets *gate = 1 when the value is below tol or *its reaches
+ * max_iter.  rule 0: sqrt(num) / sqrt(den), +inf unless both are finite and den > 0 (TVM,
+ * TVME, tvm.py:416-463); rule 1: sqrt(num) / max(sqrt(den), 1e-12), +inf unless both are
+ * finite (TVMW, tvmw.py:717-773).  hist holds max_iter values; stamp (NULL = none) receives the
+ * device wall clock when the rule closes the gate.  tr_clock_stamp writes the device wall clock
+ * to *out in stream order; tr_clock_rate gives its frequency (Hz) on the current device. ---- */
+int tr_gate_set(const int* gate);
+int tr_stop_rule(const double* sums, double tol, int max_iter, int rule, int* gate, int* its, double* hist,
+                 int64_t* stamp, void* stream);
+int tr_clock_stamp(int64_t* out, void* stream);
+int tr_clock_rate(int64_t* hz);
+
 /* ---- NUFFT plan (SigPy 0.1.27 nufft defaults: width 4, oversampling 1.25) ----
  * Replaces SigPyNufftBackend.__init__ (src/toporecon/nufft_backend.py:58-88):
  * coords[b] is a device (M_b, 3) float32 array in [z, y, x] SigPy units.  The

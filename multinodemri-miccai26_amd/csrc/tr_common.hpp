@@ -43,6 +43,16 @@ const char* last_error();
 #define TR_LAUNCH_CHECK() TR_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------------------
+// Gates of batched solves: a host thread may name an int in device memory
+// (tr_gate_set); every kernel of the solver family that thread launches
+// afterwards returns at once while that int is nonzero, so a captured graph
+// of several iterations leaves an instance's state untouched once its own
+// stopping rule has closed the gate.  Null = no gate.
+// ---------------------------------------------------------------------------
+const int* gate_current();
+__device__ __forceinline__ bool gate_closed(const int* g) { return g != nullptr && *g != 0; }
+
+// ---------------------------------------------------------------------------
 // Optional per-kernel timing: when enabled, every launch is bracketed by HIP
 // events on its own stream and credited with its algorithmic bytes (each
 // input read once, each output written once).

@@ -19,6 +19,7 @@ __device__ __forceinline__ int wrap2(int p, int g) {
 
 template <int KCI, int NCI>
 __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
+  if (gate_closed(a.gate)) return;
   constexpr int SPW = 64 / KCI;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -128,6 +129,7 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
 // added in the same (dz, dy, dx) order as k_interp, so results are identical.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_interp_v4(InterpArgs a, int G) {
+  if (gate_closed(a.gate)) return;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
@@ -229,6 +231,7 @@ __device__ __forceinline__ void acc4(float4& acc, float w, const float4& u) {
 }
 
 __global__ void __launch_bounds__(256) k_interp_grp(InterpArgs a, int G) {
+  if (gate_closed(a.gate)) return;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
@@ -476,6 +479,7 @@ __device__ __forceinline__ void cl_store(const SpreadArgs& a, int64_t cl, int k,
 // pair walk so the kernel keeps the register footprint of either.
 template <int KCI, int SORTED>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs a) {
+  if (gate_closed(a.gate)) return;
   constexpr int NP = ClLanes<KCI>::NP;
   constexpr int NPL = ClLanes<KCI>::NPL;
   const int lane = threadIdx.x & 63;
@@ -552,6 +556,7 @@ __device__ __forceinline__ void spread_segment(const SpreadArgs& a, int64_t item
 
 template <int KCI, int SORTED>
 __global__ void __launch_bounds__(256) k_spread_segments(SpreadArgs a) {
+  if (gate_closed(a.gate)) return;
   const int64_t item = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (item >= a.nseg * a.ngrp) return;
   spread_segment<KCI, SORTED>(a, item, threadIdx.x & 63);
@@ -561,6 +566,7 @@ __global__ void __launch_bounds__(256) k_spread_segments(SpreadArgs a) {
 // thread per (heavy cluster, point, channel), channel fastest.
 template <int CZ, int CY, int CX>
 __global__ void __launch_bounds__(256) k_spread_combine(SpreadArgs a) {
+  if (gate_closed(a.gate)) return;
   constexpr int NPT = CZ * CY * CX;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int K = a.K;
@@ -657,7 +663,9 @@ int count_touched(const tr_plan* p, BinData& b, hipStream_t s) {
   return kOk;
 }
 
-int launch_interp(const InterpArgs& a, hipStream_t s) {
+int launch_interp(const InterpArgs& a0, hipStream_t s) {
+  InterpArgs a = a0;
+  a.gate = gate_current();
   const int K = a.K;
   const double per_sample = a.mode == kInterpDual ? 24.0 * K
                             : a.mode == kInterpResid ? 16.0 * K
@@ -709,6 +717,7 @@ int launch_interp(const InterpArgs& a, hipStream_t s) {
 }
 
 int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
+  a.gate = gate_current();
   const int K = a.K;
   a.col_win = p->prune ? p->col_win : nullptr;
   const double G = a.col_win ? (double)p->support_points : (double)p->G;

@@ -170,6 +170,7 @@ struct FwdXArgs {
   int K;  // grid channel count (stride of a grid point's channel record)
   int E;  // echoes of this launch: img holds E images, grid channels e * C + c
   float2* grid;
+  const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };
 
 // Division of unsigned values below 2^31 by a divisor fixed at launch: one
@@ -225,6 +226,7 @@ struct LinesArgs {
   const int2* line_win;
   int win_so, win_si;
   int zero_empty;  // an empty line still stores zeros (its outputs are read later)
+  const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };
 
 struct AdjXArgs {
@@ -242,6 +244,7 @@ struct AdjXArgs {
   int E;  // echoes of this launch (at most kAdjXEchoes): grid channels e * C + c, out (E, Z, Y, X)
   float2* out;
   int accumulate;
+  const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };
 
 constexpr int kAdjXEchoes = 8;  // echo accumulators of the inverse x pass
@@ -271,6 +274,7 @@ struct InterpArgs {
   float2* q;
   float sigma;
   float one_plus_sigma;
+  const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };
 
 struct SpreadArgs {
@@ -309,6 +313,7 @@ struct SpreadArgs {
   const int* active;
   int64_t nactive;
   unsigned seg_blocks;  // leading workgroups of the cluster kernel that run the heavy segments
+  const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };
 
 // Position p of an axis of length g lies in the window (start, len) of that axis.

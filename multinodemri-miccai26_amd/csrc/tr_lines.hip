@@ -92,6 +92,7 @@ __device__ __forceinline__ void lines_two_level(const LinesArgs& a, const float2
 
 template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
+  if (gate_closed(a.gate)) return;
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
   const int L = LC > 0 ? LC : a.ax.L;   // line length
@@ -194,6 +195,7 @@ struct LinesLaunch {
 int launch_lines(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const LinesArgs& a0) {
   if (a0.K <= 0 || grid.x == 0) return kOk;  // nothing to transform (and no divisor of 0 below)
   LinesArgs a = a0;
+  a.gate = gate_current();
   const unsigned S = a.xcd_chunk;
   a.div_nchunk = make_fastdiv((unsigned)((a.K + kc - 1) / kc));
   a.div_inner = make_fastdiv((unsigned)a.n_inner);

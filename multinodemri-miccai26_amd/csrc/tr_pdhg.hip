@@ -19,6 +19,7 @@ constexpr int kEwBlock = 256;
 struct VolGeom {
   int B, E, Z, Y, X;
   int64_t N;
+  const int* gate = nullptr;  // batched solves: the launch returns at once while *gate != 0
 };
 
 __device__ __forceinline__ float2 ld(const float2* p, int64_t i) { return p[i]; }
@@ -105,6 +106,7 @@ __device__ __forceinline__ void motion_dual(const VolGeom& g, int b, int64_t v, 
 
 __global__ void __launch_bounds__(kEwBlock) k_tvm_dual(VolGeom g, const float2* ubar, float2* p_m,
                                                        float sigma, float lam, int l2) {
+  if (gate_closed(g.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
   if (t >= (int64_t)g.B * g.N) return;
   const int b = (int)(t / g.N);
@@ -118,6 +120,7 @@ __global__ void __launch_bounds__(kEwBlock) k_tvm_dual(VolGeom g, const float2* 
 __global__ void __launch_bounds__(kEwBlock) k_tvme_dual(VolGeom g, const float2* ubar, float2* p_m,
                                                         float2* p_ex, float2* p_ey, float2* p_ez,
                                                         float sigma, float lam_m, float lam_e, int l2) {
+  if (gate_closed(g.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
   if (t >= (int64_t)g.B * g.N) return;
   const int b = (int)(t / g.N);
@@ -179,6 +182,7 @@ __device__ __forceinline__ float2 dmt(const VolGeom& g, const float2* p_m, int b
 __global__ void __launch_bounds__(kEwBlock) k_tvm_primal(VolGeom g, float2* u, float2* ubar,
                                                          const float2* grad, const float2* p_m,
                                                          float tau, double2* partial) {
+  if (gate_closed(g.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
   double num = 0.0, den = 0.0;
   if (t < (int64_t)g.B * g.N) {
@@ -209,6 +213,7 @@ __global__ void __launch_bounds__(kEwBlock) k_tvme_primal(VolGeom g, float2* u, 
                                                           const float2* p_ex, const float2* p_ey,
                                                           const float2* p_ez, float tau,
                                                           double2* partial) {
+  if (gate_closed(g.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
   double num = 0.0, den = 0.0;
   if (t < (int64_t)g.B * g.N) {
@@ -255,6 +260,7 @@ __global__ void __launch_bounds__(kEwBlock) k_tvme_primal(VolGeom g, float2* u, 
 __global__ void __launch_bounds__(kEwBlock) k_tvmw_dual(VolGeom g, const float2* ubar, float2* p_m,
                                                         float2* p_w1, float sigma, float lam1,
                                                         float lam2, int l2) {
+  if (gate_closed(g.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
   if (t >= (int64_t)g.B * g.N) return;
   const int b = (int)(t / g.N);
@@ -290,6 +296,7 @@ __global__ void __launch_bounds__(kEwBlock) k_tvmw_primal(VolGeom g, float2* u, 
                                                           const float2* grad, const float2* p_m,
                                                           const float2* p_w1, const float2* divw2,
                                                           float tau, double2* partial) {
+  if (gate_closed(g.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * kEwBlock + threadIdx.x;
   double num = 0.0, den = 0.0;
   if (t < (int64_t)g.B * g.N) {
@@ -374,7 +381,9 @@ __global__ void __launch_bounds__(kEwBlock) k_cdot_re(const float2* a, const flo
   block_sum1(acc, partial);
 }
 
-__global__ void __launch_bounds__(kEwBlock) k_reduce_partials(const double2* partial, int n, double* out) {
+__global__ void __launch_bounds__(kEwBlock) k_reduce_partials(const double2* partial, int n, double* out,
+                                                              const int* gate) {
+  if (gate_closed(gate)) return;
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < n; i += kEwBlock) {
     a += partial[i].x;
@@ -405,6 +414,7 @@ static int check_geom(int B, int E, int Z, int Y, int X, VolGeom* g) {
   g->Y = Y;
   g->X = X;
   g->N = (int64_t)Z * Y * X;
+  g->gate = gate_current();
   return kOk;
 }
 
@@ -579,7 +589,8 @@ int tr_tvme_primal_noalloc(int B, int E, int Z, int Y, int X, tr_c64* u, tr_c64*
     TR_LAUNCH_CHECK();
   }
   ProfScope ps(kKReduce, 16.0 * n + 16.0, s);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kEwBlock), 0, s, (const double2*)part, (int)n, sums);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kEwBlock), 0, s, (const double2*)part, (int)n, sums,
+                     gate_current());
   TR_LAUNCH_CHECK();
   return kOk;
 }
@@ -588,7 +599,7 @@ int tr_reduce_partials(const double* partial, int64_t n, double* out2, void* str
   if (n < 1 || n > INT32_MAX) TR_FAIL(kInvalidArgument, "bad partial count");
   ProfScope ps(kKReduce, 16.0 * n + 16.0, (hipStream_t)stream);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kEwBlock), 0, (hipStream_t)stream,
-                     (const double2*)partial, (int)n, out2);
+                     (const double2*)partial, (int)n, out2, gate_current());
   TR_LAUNCH_CHECK();
   return kOk;
 }

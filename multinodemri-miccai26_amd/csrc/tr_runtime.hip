@@ -10,6 +10,31 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
 
+static thread_local const int* g_gate = nullptr;
+const int* gate_current() { return g_gate; }
+
+// The stopping rule of one instance on the device (one thread).
+__global__ void k_stop_rule(const double* sums, double tol, int max_iter, int rule, int* gate, int* its,
+                            double* hist, int64_t* stamp) {
+  if (*gate) return;
+  const double num = sums[0], den = sums[1];
+  double v = HUGE_VAL;
+  if (rule == 0) {
+    if (isfinite(num) && isfinite(den) && den > 0.0) v = sqrt(fmax(num, 0.0)) / sqrt(den);
+  } else if (isfinite(num) && isfinite(den)) {
+    v = sqrt(fmax(num, 0.0)) / fmax(sqrt(fmax(den, 0.0)), 1e-12);
+  }
+  const int c = *its + 1;
+  *its = c;
+  hist[c - 1] = v;
+  if (v < tol || c >= max_iter) {
+    *gate = 1;
+    if (stamp) *stamp = (int64_t)wall_clock64();
+  }
+}
+
+__global__ void k_clock_stamp(int64_t* out) { *out = (int64_t)wall_clock64(); }
+
 // ---- kernel timing -----------------------------------------------------------
 namespace {
 struct ProfPending {
@@ -62,6 +87,8 @@ const char* kernel_name(int id) {
 
 ProfScope::ProfScope(int id, double bytes, hipStream_t s) : id_(id), bytes_(bytes), s_(s), start_(nullptr) {
   if (!g_prof_on) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;  // not in graphs
   std::lock_guard<std::mutex> lk(g_prof_mu);
   hipEvent_t e = pool_get();
   if (e && hipEventRecord(e, s) == hipSuccess) start_ = e;
@@ -151,6 +178,38 @@ int tr_graph_destroy(tr_graph* g) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
   }
+  return kOk;
+}
+
+int tr_gate_set(const int* gate) {
+  g_gate = gate;
+  return kOk;
+}
+
+int tr_stop_rule(const double* sums, double tol, int max_iter, int rule, int* gate, int* its, double* hist,
+                 int64_t* stamp, void* stream) {
+  if (!sums || !gate || !its || !hist) TR_FAIL(kInvalidArgument, "null argument");
+  if (max_iter < 1) TR_FAIL(kInvalidArgument, "max_iter must be positive");
+  if (rule != 0 && rule != 1) TR_FAIL(kInvalidArgument, "rule is 0 (TVM, TVME) or 1 (TVMW)");
+  hipLaunchKernelGGL(k_stop_rule, dim3(1), dim3(1), 0, (hipStream_t)stream, sums, tol, max_iter, rule, gate, its,
+                     hist, stamp);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_clock_stamp(int64_t* out, void* stream) {
+  if (!out) TR_FAIL(kInvalidArgument, "null argument");
+  hipLaunchKernelGGL(k_clock_stamp, dim3(1), dim3(1), 0, (hipStream_t)stream, out);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_clock_rate(int64_t* hz) {
+  if (!hz) TR_FAIL(kInvalidArgument, "null argument");
+  int dev = 0, khz = 0;
+  TR_HIP(hipGetDevice(&dev));
+  TR_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  *hz = (int64_t)khz * 1000;
   return kOk;
 }
 

@@ -35,11 +35,13 @@ struct AxisArgs {
   int64_t sig_stride;  // complex elements per image of the signal array
   int64_t lo_stride, hi_stride;
   int64_t lo_off, hi_off;
+  const int* gate = nullptr;  // batched solves: the launch returns at once while *gate != 0
 };
 
 // Analysis along one axis: lo[k] = sum_j h[j] x[2k + 1 - j], zeros outside [0, n).
 __global__ void k_dwt_axis(AxisArgs a, const float2* __restrict__ x, float2* __restrict__ lo,
                            float2* __restrict__ hi) {
+  if (gate_closed(a.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = a.rows * a.c * a.Q;
   if (t >= total) return;
@@ -74,6 +76,7 @@ __global__ void k_dwt_axis(AxisArgs a, const float2* __restrict__ x, float2* __r
 // x[m] = sum_k lo[k] h_lo[2k + 1 - m] + hi[k] h_hi[2k + 1 - m].
 __global__ void k_idwt_axis(AxisArgs a, const float2* __restrict__ lo, const float2* __restrict__ hi,
                             float2* __restrict__ x) {
+  if (gate_closed(a.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = a.rows * a.n * a.Q;
   if (t >= total) return;
@@ -111,7 +114,8 @@ __global__ void k_idwt_axis(AxisArgs a, const float2* __restrict__ lo, const flo
 
 // p <- proj(p + sigma w): t * min(1, lam / sqrt(re^2 + im^2 + 1e-12)); 0 if lam <= 0
 // (tvmw.py:162-193 tree axpy + L-inf projection of every sub-band).
-__global__ void k_w2_dual(float2* p, const float2* w, int64_t n, float sigma, float lam) {
+__global__ void k_w2_dual(float2* p, const float2* w, int64_t n, float sigma, float lam, const int* gate) {
+  if (gate_closed(gate)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (lam <= 0.f) {
@@ -171,7 +175,9 @@ static int make_levels(int Z, int Y, int X, int level, std::vector<Level>& lv, i
   return kOk;
 }
 
-static int launch_axis(const AxisArgs& a, const float2* x, float2* lo, float2* hi, hipStream_t s) {
+static int launch_axis(const AxisArgs& a0, const float2* x, float2* lo, float2* hi, hipStream_t s) {
+  AxisArgs a = a0;
+  a.gate = gate_current();
   const int64_t total = a.rows * a.c * a.Q;
   if (total == 0) return kOk;
   hipLaunchKernelGGL(k_dwt_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, x, lo, hi);
@@ -179,7 +185,9 @@ static int launch_axis(const AxisArgs& a, const float2* x, float2* lo, float2* h
   return kOk;
 }
 
-static int launch_iaxis(const AxisArgs& a, const float2* lo, const float2* hi, float2* x, hipStream_t s) {
+static int launch_iaxis(const AxisArgs& a0, const float2* lo, const float2* hi, float2* x, hipStream_t s) {
+  AxisArgs a = a0;
+  a.gate = gate_current();
   const int64_t total = a.rows * a.n * a.Q;
   if (total == 0) return kOk;
   hipLaunchKernelGGL(k_idwt_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, lo, hi, x);
@@ -358,7 +366,7 @@ int tr_w2_dual_update(tr_c64* p, const tr_c64* w, int64_t n, float sigma, float 
   hipStream_t s = (hipStream_t)stream;
   ProfScope ps(kKW2Dual, 24.0 * n, s);
   hipLaunchKernelGGL(k_w2_dual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, F2(p), F2(w), n, sigma,
-                     lam);
+                     lam, gate_current());
   TR_LAUNCH_CHECK();
   return kOk;
 }

@@ -13,6 +13,7 @@ namespace tr {
 // ---------------------------------------------------------------------------
 template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
+  if (gate_closed(a.gate)) return;
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
   const int LF = LC > 0 ? LC : a.ax.M;  // transform length (tile rows, twiddles)
@@ -138,6 +139,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
 // ---------------------------------------------------------------------------
 template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
+  if (gate_closed(a.gate)) return;
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
   constexpr int kSlots = (max_line(KC) + kBlock - 1) / kBlock;
@@ -283,6 +285,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
 // ---------------------------------------------------------------------------
 template <int KC, int LC>
 __global__ void __launch_bounds__(kBlock) k_adj_x_channels(AdjXArgs a) {
+  if (gate_closed(a.gate)) return;
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
   const int LF = LC > 0 ? LC : a.ax.M;
@@ -339,7 +342,9 @@ struct AdjXLaunch {
   }
 };
 
-int launch_fwd_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const FwdXArgs& a) {
+int launch_fwd_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const FwdXArgs& a0) {
+  FwdXArgs a = a0;
+  a.gate = gate_current();
   return launch_fft_kernel<FwdXLaunch>(kc, L, grid, smem, s, a);
 }
 
@@ -350,11 +355,15 @@ struct AdjXChannelsLaunch {
   }
 };
 
-int launch_adj_x_channels(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a) {
+int launch_adj_x_channels(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a0) {
+  AdjXArgs a = a0;
+  a.gate = gate_current();
   return launch_fft_kernel<AdjXChannelsLaunch>(kc, L, grid, smem, s, a);
 }
 
-int launch_adj_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a) {
+int launch_adj_x(int kc, int L, dim3 grid, size_t smem, hipStream_t s, const AdjXArgs& a0) {
+  AdjXArgs a = a0;
+  a.gate = gate_current();
   return launch_fft_kernel<AdjXLaunch>(kc, L, grid, smem, s, a);
 }
 

@@ -32,6 +32,10 @@ SIGNATURES = {
     "tr_graph_end": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "tr_graph_launch": (c_int, [c_vp, c_vp]),
     "tr_graph_destroy": (c_int, [c_vp]),
+    "tr_gate_set": (c_int, [c_vp]),
+    "tr_stop_rule": (c_int, [c_vp, ctypes.c_double, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "tr_clock_stamp": (c_int, [c_vp, c_vp]),
+    "tr_clock_rate": (c_int, [ctypes.POINTER(c_i64)]),
     "tr_plan_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int,
                                ctypes.POINTER(c_vp), ctypes.POINTER(c_i64), c_vp]),
     "tr_plan_destroy": (c_int, [c_vp]),

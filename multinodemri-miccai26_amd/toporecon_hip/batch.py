@@ -1,4 +1,4 @@
-"""Many independent PDHG instances advanced together, one HIP-graph launch per iteration.
+"""Many independent PDHG instances advanced together in HIP graphs.
 
 The reference reconstructs one problem per invocation and launches every kernel
 from Python in every iteration (``tvm.py:390-491``).  Here a set of independent,
@@ -16,6 +16,19 @@ bit-identical to ``solver.run()``.
 mode="streams" issues the same per-instance iteration eagerly, each instance on
 its own stream (so independent instances can share the GPU through its hardware
 queues), and reads all stopping sums after one synchronisation per iteration.
+
+mode="persistent" captures one graph, once, for the whole solve: `chunk`
+iterations of every instance, each iteration followed by the instance's stopping
+rule evaluated on the device (``tr_stop_rule``), every launch of an instance gated
+by its own device flag (``tr_gate_set``).  The rule closes the gate when the
+instance meets its tolerance or its iteration cap, and every later kernel of that
+instance returns at once, so its state stays exactly where a solve alone would
+have stopped.  The graph is replayed unchanged until every gate is closed (one
+host synchronisation per replay; with chunk >= the largest iteration cap a single
+launch runs the whole batch); no instance waits for another between replays, so
+small instances run beside large ones.  Iteration counts and histories come from
+the device; per-instance times from device clock stamps taken when each gate
+closes.
 """
 
 from __future__ import annotations
@@ -83,16 +96,96 @@ class GraphBatch:
                     self.solvers[i].pdhg_launch(0)
                 self.origin.wait_stream(st)
 
-    def run(self, on_replay=None, mode="graph"):
+    def _capture_persistent(self, chunk, gates, its, hist, stamps, rules, maxit):
+        import torch
+        lib = _lib.load()
+        h = self.origin.cuda_stream
+        torch.cuda.synchronize(self.device)
+        graph = ctypes.c_void_p()
+        g0, i0, h0, s0 = gates.data_ptr(), its.data_ptr(), hist.data_ptr(), stamps.data_ptr()
+        with torch.cuda.device(self.device), torch.cuda.stream(self.origin):
+            _lib.call("tr_graph_begin", h)
+            try:
+                _lib.call("tr_clock_stamp", s0, h)  # stamps[0]: start of the replay
+                for st in self.streams:
+                    st.wait_stream(self.origin)
+                for _ in range(chunk):
+                    for i, s in enumerate(self.solvers):
+                        st = self.streams[i]
+                        with torch.cuda.stream(st):
+                            _lib.call("tr_gate_set", g0 + 4 * i)
+                            try:
+                                s.pdhg_launch(0)
+                                _lib.call("tr_stop_rule", s._red.data_ptr(), float(s.tol), int(s.max_iter),
+                                          rules[i], g0 + 4 * i, i0 + 4 * i, h0 + 8 * maxit * i, s0 + 8 * (i + 1),
+                                          st.cuda_stream)
+                            finally:
+                                _lib.call("tr_gate_set", None)
+                for st in self.streams:
+                    self.origin.wait_stream(st)
+            finally:
+                rc = lib.tr_graph_end(h, ctypes.byref(graph))
+        _lib.check(rc, "tr_graph_end")
+        self.captures += 1
+        return graph
+
+    def _run_persistent(self, chunk):
+        import torch
+        dev = self.device
+        n = len(self.solvers)
+        maxit = max(int(s.max_iter) for s in self.solvers)
+        chunk = maxit if chunk is None else max(1, min(int(chunk), maxit))
+        gates = torch.zeros(n, dtype=torch.int32, device=dev)
+        its = torch.zeros(n, dtype=torch.int32, device=dev)
+        hist = torch.zeros((n, maxit), dtype=torch.float64, device=dev)
+        stamps = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+        rules = [1 if getattr(s, "algorithm", "") == "tvmw" else 0 for s in self.solvers]
+        self._prepare()
+        hz = ctypes.c_int64()
+        with torch.cuda.device(dev):
+            _lib.call("tr_clock_rate", ctypes.byref(hz))
+        graph = self._capture_persistent(chunk, gates, its, hist, stamps, rules, maxit)
+        lib = _lib.load()
+        self.replays = 0
+        t_first = None
+        try:
+            while True:
+                _lib.call("tr_graph_launch", graph, self.origin.cuda_stream)
+                self.replays += 1
+                self.origin.synchronize()
+                if t_first is None:
+                    t_first = int(stamps[0].item())  # each replay restamps; the first one starts the clock
+                if bool(torch.all(gates != 0)):
+                    break
+                if self.replays * chunk >= maxit:
+                    raise RuntimeError("batched solve did not close every gate within the iteration caps")
+        finally:
+            lib.tr_graph_destroy(graph)
+        its_h = its.tolist()
+        hist_h = hist.cpu().numpy()
+        st_h = stamps.tolist()
+        out = []
+        for i, s in enumerate(self.solvers):
+            s.iterations = int(its_h[i])
+            s.history = [float(v) for v in hist_h[i, :s.iterations]]
+            s.elapsed = (st_h[i + 1] - t_first) / float(hz.value)
+            out.append({"iterations": s.iterations, "seconds": s.elapsed, "history": s.history})
+        return out
+
+    def run(self, on_replay=None, mode="graph", chunk=10):
         """Advance every instance to its stopping rule.  Returns per-instance dicts with the
         iteration count, the seconds until it finished (from the first replay) and its history;
-        each solver's ``ubar`` holds its result.  on_replay(k, active) runs after replay k."""
+        each solver's ``ubar`` holds its result.  on_replay(k, active) runs after replay k (graph
+        and streams modes); chunk is the iterations per replay of the persistent mode (None: the
+        largest iteration cap, i.e. one launch)."""
         import torch
-        if mode not in ("graph", "streams"):
+        if mode not in ("graph", "streams", "persistent"):
             raise ValueError(f"unknown mode {mode!r}")
-        if mode == "graph" and any(getattr(s, "_bstreams", None) for s in self.solvers):
+        if mode != "streams" and any(getattr(s, "_bstreams", None) for s in self.solvers):
             # per-bin streams forked inside a capture are not supported: set_bin_streams(1) first
-            raise ValueError("graph mode needs every instance on one bin stream (set_bin_streams(1))")
+            raise ValueError(f"{mode} mode needs every instance on one bin stream (set_bin_streams(1))")
+        if mode == "persistent":
+            return self._run_persistent(chunk)
         self._prepare()
         n = len(self.solvers)
         its = [0] * n

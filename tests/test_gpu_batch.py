@@ -98,3 +98,44 @@ def test_graph_mode_refuses_per_bin_streams(torch_mod):
     s.set_bin_streams(2)
     with pytest.raises(ValueError):
         GraphBatch([s]).run(mode="graph")
+
+
+@pytest.mark.parametrize("chunk", [4, None])
+def test_persistent_batch_matches_solves_alone(torch_mod, chunk):
+    """One graph captured once: every instance's iteration gated by its own device flag and
+    followed by its stopping rule on the device.  Instances with different caps and
+    tolerances stop mid-replay (chunk 4) or inside a single launch (chunk None); each final
+    image, iteration count and stopping history equals the instance solved alone."""
+    torch = torch_mod
+    from toporecon_hip.batch import GraphBatch
+    specs = [("tvm", dict(shape=(8, 10, 12), seed=31), 6, 0.0),
+             ("tvme", dict(shape=(10, 12, 12), coils=2, echoes=3, bins=2, seed=32), 9, 0.0),
+             ("tvmw", dict(shape=(12, 12, 16), seed=33, algorithm="tvmw"), 5, 0.0),
+             ("tvm", dict(shape=(8, 12, 12), seed=41), 100, 1e-2),
+             ("tvm", dict(shape=(12, 16, 16), seed=42), 100, 3e-3)]
+    solvers, ref = [], []
+    for alg, kw, n, tol in specs:
+        acq, ksp, _ = make_problem(**kw)
+        s = _solver(alg, acq, ksp, n, tol)
+        u = s.run().clone()
+        ref.append((u, s.iterations, list(s.history)))
+        solvers.append(s)
+    assert ref[3][1] < 100 and ref[4][1] < 100, [r[1] for r in ref]  # the tolerance stops them
+    batch = GraphBatch(solvers)
+    out = batch.run(mode="persistent", chunk=chunk)
+    assert batch.captures == 1
+    assert batch.replays == (1 if chunk is None else -(-max(r[1] for r in ref) // chunk))
+    for s, (u, n, hist), o in zip(solvers, ref, out):
+        assert o["iterations"] == n
+        assert o["history"] == hist  # float64 values of the device rule equal the host's
+        assert torch.equal(s.ubar, u)
+        assert o["seconds"] > 0
+
+
+def test_persistent_batch_refuses_per_bin_streams(torch_mod):
+    from toporecon_hip.batch import GraphBatch
+    acq, ksp, _ = make_problem(shape=(8, 10, 12), bins=3, seed=35)
+    s = _solver("tvm", acq, ksp, 2, 0.0)
+    s.set_bin_streams(2)
+    with pytest.raises(ValueError):
+        GraphBatch([s]).run(mode="persistent")
