@@ -85,6 +85,16 @@ int tr_clock_rate(int64_t* hz);
  * builds the (grid cluster, sample) lists of the deterministic adjoint. */
 int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
                    const int64_t* samples_per_bin, void* stream);
+/* tr_plan_create with the sample geometry chosen by the caller (no reference counterpart: SigPy
+ * has no gridding tiles; an autotuner picks them).  sort_tile (2 .. 32, default 8): edge of the
+ * grid tiles samples are ordered by (then by first tap inside the tile), which sets which
+ * samples the interpolation gathers together and the order of every cluster's pair list.
+ * spread_segment (64 .. 65536, default 1024): pair lists of the spreading longer than this are
+ * cut into segments of that many pairs whose partial sums are added in segment order.  Results
+ * are deterministic for each geometry; two geometries add the same products in different
+ * orders (float32 rounding). */
+int tr_plan_create_geom(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
+                        const int64_t* samples_per_bin, int sort_tile, int spread_segment, void* stream);
 int tr_plan_destroy(tr_plan* plan);
 /* info (12 values): [0..2] image dims, [3..5] grid dims, [6] bins, [7] M_b, [8] nnz_b, [9]
  * channels of the workspace last used, [10] distinct grid points under bin b's taps, [11] grid
@@ -128,17 +138,22 @@ int tr_plan_release_workspace(tr_plan* plan);
  * longest list first, index order inside a brick; 0: every cluster in index order.  Results are
  * identical for every order. */
 #define TR_PARAM_SPREAD_ORDER 10
+/* TR_PARAM_SORT_TILE, TR_PARAM_SPREAD_SEGMENT: read-only, the geometry the plan was built with. */
+#define TR_PARAM_SORT_TILE 11
+#define TR_PARAM_SPREAD_SEGMENT 12
 int tr_plan_set_param(tr_plan* plan, int param, int64_t value);
 int tr_plan_get_param(const tr_plan* plan, int param, int64_t* value);
 
 /* Sample-order helpers for the fused solver path (no reference counterpart:
  * the reference keeps samples in trajectory order).  gather: (K, M_b) original
- * order -> (M_b, K) sorted; scatter: inverse; gather_weights: (M_b) -> (M_b). */
+ * order -> (M_b, K) sorted; scatter: inverse; gather_weights: (M_b) -> (M_b), scatter_weights its
+ * inverse. */
 int tr_plan_gather_samples(const tr_plan* plan, int bin, const tr_c64* src, int K, tr_c64* dst,
                            void* stream);
 int tr_plan_scatter_samples(const tr_plan* plan, int bin, const tr_c64* src, int K, tr_c64* dst,
                             void* stream);
 int tr_plan_gather_weights(const tr_plan* plan, int bin, const float* src, float* dst, void* stream);
+int tr_plan_scatter_weights(const tr_plan* plan, int bin, const float* src, float* dst, void* stream);
 
 /* Forward NUFFT of E*C images of one bin: out[(e*C + c), m] = nufft(img[e] * maps[c]).
  * Replaces SigPyNufftBackend.forward -> sp.nufft (src/toporecon/nufft_backend.py:101-107);

@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(256) k_interp_grp(InterpArgs a, int G) {
 // points.  A wave owns one cluster and KCI channels (lanes: KCI channels x
 // 64/KCI point slots) and keeps the cluster's sums in registers while it walks
 // the cluster's (sample, weights) list once, in sorted-sample order.  Lists
-// longer than kSpreadSeg pairs (the k-space centre, where every interleave
+// longer than the plan's segment length (the k-space centre, where every interleave
 // passes) are cut at plan time into fixed segments reduced by separate waves;
 // their partial sums are then added in segment order, so the result depends
 // only on the plan, never on the launch configuration or scheduling.
@@ -473,7 +473,7 @@ __device__ __forceinline__ void cl_store(const SpreadArgs& a, int64_t cl, int k,
   }
 }
 
-// Clusters with at most kSpreadSeg pairs (empty clusters are written as 0).
+// Clusters with at most a.seg pairs (empty clusters are written as 0).
 // With a.seg_blocks > 0 the first seg_blocks workgroups run the heavy-cluster segments (the
 // longest waves first, their sums to a.partial), the rest the light clusters; both share one
 // pair walk so the kernel keeps the register footprint of either.
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs a) {
     grp = (int)((unsigned)item - usg * (unsigned)a.ngrp);
     cl = a.seg_row[sg];
     j0 = a.seg_j0[sg];
-    j1 = min(j0 + kSpreadSeg, a.row_ptr[cl + 1]);
+    j1 = min(j0 + a.seg, a.row_ptr[cl + 1]);
   } else {
     blk -= a.seg_blocks;
     if (blk < a.xcd_full) {  // runs of xcd_chunk consecutive logical blocks per XCD
@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs a) {
     cl = a.active ? (unsigned)a.active[uit] : uit;
     j0 = a.row_ptr[cl];
     j1 = a.row_ptr[cl + 1];
-    if (!a.active && j1 - j0 > kSpreadSeg) return;  // heavy cluster: written by k_spread_combine
+    if (!a.active && j1 - j0 > a.seg) return;  // heavy cluster: written by k_spread_combine
   }
   const int k = grp * KCI + lane % KCI;
   const int pl = lane / KCI;
@@ -539,7 +539,7 @@ __device__ __forceinline__ void spread_segment(const SpreadArgs& a, int64_t item
   const int grp = (int)(item - sg * a.ngrp);
   const int cl = a.seg_row[sg];
   const int j0 = a.seg_j0[sg];
-  const int j1 = min(j0 + kSpreadSeg, a.row_ptr[cl + 1]);
+  const int j1 = min(j0 + a.seg, a.row_ptr[cl + 1]);
   const int k = grp * KCI + lane % KCI;
   const int pl = lane / KCI;
   float2 acc[NP];
@@ -729,6 +729,7 @@ int launch_spread(SpreadArgs a, const BinData& b, tr_plan* p, hipStream_t s) {
   a.NCL = p->NCL;
   a.row_ptr = b.row_ptr;
   a.pairs = b.pairs;
+  a.seg = p->seg;
   a.ngrp = (K + kci - 1) / kci;
   if (a.NCL * a.ngrp >= ((int64_t)1 << 31)) TR_FAIL(kUnsupported, "too many grid clusters for the spreading");
   a.div_ngrp = make_fastdiv((unsigned)a.ngrp);

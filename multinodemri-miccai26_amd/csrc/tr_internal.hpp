@@ -21,12 +21,14 @@
 namespace tr {
 
 constexpr int kWidth = 4;
-constexpr int kTile = 8;  // sample sort tile edge (grid points)
+constexpr int kTile = 8;  // default sample sort tile edge (grid points)
+constexpr int kMinTile = 2, kMaxTile = 32;
 constexpr int kMaxKc = 16;  // channels per FFT workgroup
 #ifndef TR_SPREAD_SEG
 #define TR_SPREAD_SEG 1024
 #endif
-constexpr int kSpreadSeg = TR_SPREAD_SEG;  // pairs per heavy-cluster segment
+constexpr int kSpreadSeg = TR_SPREAD_SEG;  // default pairs per heavy-cluster segment
+constexpr int kMinSpreadSeg = 64, kMaxSpreadSeg = 1 << 16;
 constexpr int kSpreadOrderLongestFirst = 0xffff;  // TR_PARAM_SPREAD_ORDER code of the default order
 constexpr int kSpreadOrderBrickLongestFirst = 0x10000;  // + brick code: bricks, longest list's brick first
 #ifndef TR_SPREAD_UNROLL
@@ -133,6 +135,11 @@ struct tr_plan {
   int lines_xcd_lines = 4;  // y / z passes: whole lines per XCD run (0 = round-robin)
   int interp_kgroups = 0;  // 1: one 64-channel group per launch row (group-major order)
   int spread_order = 0;    // 0: all clusters in index order; else the code of the active-cluster order
+  // sample geometry, fixed when the plan is built: the sort tile edge (samples are ordered by
+  // tile, then by first tap inside the tile) and the heavy-cluster segment length (pair lists
+  // longer than it are cut into segments of that many pairs, reduced in segment order)
+  int tile = tr::kTile;
+  int seg = tr::kSpreadSeg;
   // k-space support of all bins' taps: per (y, x) column of the oversampled
   // grid (natural order) the window (start, len) of z positions any tap
   // touches; len 0 = empty column.  Grid points outside it are never read by
@@ -313,6 +320,7 @@ struct SpreadArgs {
   const int* active;
   int64_t nactive;
   unsigned seg_blocks;  // leading workgroups of the cluster kernel that run the heavy segments
+  int seg;              // pairs per heavy-cluster segment (the plan's)
   const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };
 

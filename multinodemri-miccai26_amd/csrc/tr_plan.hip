@@ -14,7 +14,7 @@ namespace tr {
 // ---------------------------------------------------------------------------
 // Plan construction kernels
 // ---------------------------------------------------------------------------
-__global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh, int3 g, int3 nt,
+__global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh, int3 g, int3 nt, int tile_edge,
                              float4* kp, unsigned long long* key, int* idx) {
   // k' = fl32(fl32(k * s) + shift), as SigPy's _scale_coord on a float32 trajectory:
   // the product must be rounded before the add (no FMA), or a coordinate that
@@ -29,7 +29,7 @@ __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh,
   const float ky = py + sh.y;
   const float kx = px + sh.z;
   kp[m] = make_float4(kz, ky, kx, 0.f);
-  // Key = (kTile^3 tile of the first tap, first tap within the tile, 5-tap flags), so samples
+  // Key = (tile_edge^3 tile of the first tap, first tap within the tile, 5-tap flags), so samples
   // with the same taps (same first tap and tap counts, as axis_taps computes them) are adjacent
   // in the sorted order and the interpolation can gather their taps once.
   unsigned long long tile = 0;
@@ -44,18 +44,19 @@ __global__ void k_scale_keys(const float* coord, int64_t M, float3 s, float3 sh,
       t0 = (int)ceil(kd - 2.0);
       n = (int)floor(kd + 2.0) - t0 + 1;
     }
-    int t = (int)floor((t0 + 4) / (double)kTile);
+    int t = (int)floor((t0 + 4) / (double)tile_edge);
     t = t < 0 ? 0 : (t >= ntv[ax] ? ntv[ax] - 1 : t);
-    const int off = t0 + 4 - t * kTile;  // in [0, kTile) unless clamped
+    const int off = t0 + 4 - t * tile_edge;  // in [0, tile_edge) unless clamped
     tq[ax] = (unsigned)t;
-    local = local * (unsigned)(kTile + 2) + (unsigned)(off < 0 ? 0 : (off > kTile ? kTile + 1 : off));
+    local = local * (unsigned)(tile_edge + 2) + (unsigned)(off < 0 ? 0 : (off > tile_edge ? tile_edge + 1 : off));
     five = five * 2u + (n == 5 ? 1u : 0u);
   }
   // tiles in Z order (bits of z, y, x interleaved), so runs of consecutive tiles cover compact
   // blocks of the grid rather than rows
   for (int bit = 9; bit >= 0; --bit)
     tile = (tile << 3) | (((tq[0] >> bit) & 1u) << 2) | (((tq[1] >> bit) & 1u) << 1) | ((tq[2] >> bit) & 1u);
-  key[m] = (tile * (unsigned long long)((kTile + 2) * (kTile + 2) * (kTile + 2)) + local) * 8ull + five;
+  const unsigned long long lw = (unsigned long long)(tile_edge + 2);
+  key[m] = (tile * (lw * lw * lw) + local) * 8ull + five;
   idx[m] = (int)m;
 }
 
@@ -391,21 +392,25 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
   TR_HIP(hipMalloc(&b.kp, sizeof(float4) * M));
   TR_HIP(hipMalloc(&b.tap, sizeof(TapRec) * M));
   int3 nt;
-  nt.x = p->g[0] / kTile + 2;
-  nt.y = p->g[1] / kTile + 2;
-  nt.z = p->g[2] / kTile + 2;
+  const int te = p->tile;
+  nt.x = p->g[0] / te + 2;
+  nt.y = p->g[1] / te + 2;
+  nt.z = p->g[2] / te + 2;
   const int3 g3 = make_int3(p->g[0], p->g[1], p->g[2]);
   const int3 ncl3 = make_int3(p->ncl[0], p->ncl[1], p->ncl[2]);
   hipLaunchKernelGGL(k_scale_keys, dim3(grid1d(M, 256)), dim3(256), 0, s, coord, M,
                      make_float3(p->coord_scale[0], p->coord_scale[1], p->coord_scale[2]),
-                     make_float3(p->coord_shift[0], p->coord_shift[1], p->coord_shift[2]), g3, nt,
+                     make_float3(p->coord_shift[0], p->coord_shift[1], p->coord_shift[2]), g3, nt, te,
                      kp_raw, key_in, idx);
   TR_LAUNCH_CHECK();
   int key_bits = 1;
   const int64_t zspan = (int64_t)pow2ceil(std::max(nt.x, std::max(nt.y, nt.z)));  // Z-order tile range per axis
   if (zspan > 1024) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
-  while (((int64_t)1 << key_bits) < zspan * zspan * zspan * (kTile + 2) * (kTile + 2) * (kTile + 2) * 8)
-    ++key_bits;
+  // the key's range, zspan^3 tiles x (te + 2)^3 offsets x 8 flags, in bits (a product of powers)
+  int zbits = 0, lbits = 0;
+  while (((int64_t)1 << zbits) < zspan) ++zbits;
+  while (((int64_t)1 << lbits) < (int64_t)(te + 2) * (te + 2) * (te + 2)) ++lbits;
+  key_bits = 3 * zbits + lbits + 3;
   if (key_bits > 64) TR_FAIL(kUnsupported, "grid too large for the sample sort key");
   size_t tmp_bytes = 0;
   TR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key_in, key_out, idx, b.perm, (int)M, 0,
@@ -469,16 +474,16 @@ int build_bin(tr_plan* p, BinData& b, const float* coord, int64_t M, hipStream_t
                      b.row_ptr);
   TR_LAUNCH_CHECK();
   TR_HIP(hipStreamSynchronize(s));
-  {  // heavy clusters -> fixed segments of kSpreadSeg pairs
+  {  // heavy clusters -> fixed segments of p->seg pairs
     std::vector<int> rp(p->NCL + 1);
     TR_HIP(hipMemcpy(rp.data(), b.row_ptr, sizeof(int) * (p->NCL + 1), hipMemcpyDeviceToHost));
     std::vector<int> seg_row, seg_j0, heavy_row, heavy_ptr;
     for (int64_t t = 0; t < p->NCL; ++t) {
       const int len = rp[t + 1] - rp[t];
-      if (len <= kSpreadSeg) continue;
+      if (len <= p->seg) continue;
       heavy_row.push_back((int)t);
       heavy_ptr.push_back((int)seg_row.size());
-      for (int j = rp[t]; j < rp[t + 1]; j += kSpreadSeg) {
+      for (int j = rp[t]; j < rp[t + 1]; j += p->seg) {
         seg_row.push_back((int)t);
         seg_j0.push_back(j);
       }
@@ -610,7 +615,7 @@ int build_cluster_order(tr_plan* p, BinData& b, int code) {
             for (int cx = x0; cx < std::min(x0 + bx, ncx); ++cx) {
               const int64_t t = ((int64_t)cz * ncy + cy) * ncx + cx;
               const int len = rp[t + 1] - rp[t];
-              if (len > kSpreadSeg) continue;
+              if (len > p->seg) continue;
               bool live = len > 0 || win.empty();
               for (int iy = 0; iy < kClY && !live; ++iy)
                 for (int ix = 0; ix < kClX && !live; ++ix) {
@@ -762,12 +767,25 @@ extern "C" {
 
 int tr_plan_create(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
                    const int64_t* samples_per_bin, void* stream) {
+  return tr_plan_create_geom(out, nz, ny, nx, nbins, coords, samples_per_bin, kTile, kSpreadSeg, stream);
+}
+
+int tr_plan_create_geom(tr_plan** out, int nz, int ny, int nx, int nbins, const float* const* coords,
+                        const int64_t* samples_per_bin, int sort_tile, int spread_segment, void* stream) {
   if (!out) TR_FAIL(kInvalidArgument, "null output pointer");
   *out = nullptr;
   if (nz < 1 || ny < 1 || nx < 1) TR_FAIL(kInvalidArgument, "img_shape must be three positive values");
   if (nbins < 1) TR_FAIL(kInvalidArgument, "At least one trajectory bin is required.");
+  if (sort_tile < kMinTile || sort_tile > kMaxTile)
+    TR_FAIL(kInvalidArgument, "sort tile edge outside [" + std::to_string(kMinTile) + ", " +
+                                  std::to_string(kMaxTile) + "]");
+  if (spread_segment < kMinSpreadSeg || spread_segment > kMaxSpreadSeg)
+    TR_FAIL(kInvalidArgument, "spreading segment length outside [" + std::to_string(kMinSpreadSeg) + ", " +
+                                  std::to_string(kMaxSpreadSeg) + "]");
   hipStream_t s = (hipStream_t)stream;
   tr_plan* p = new tr_plan();
+  p->tile = sort_tile;
+  p->seg = spread_segment;
   const int n[3] = {nz, ny, nx};
   const double beta = M_PI * std::sqrt(std::pow((kWidth / 1.25) * (1.25 - 0.5), 2) - 0.8);
   p->beta = (float)beta;
@@ -932,6 +950,8 @@ int tr_plan_set_param(tr_plan* p, int param, int64_t value) {
     }
     case TR_PARAM_SUPPORT_POINTS:
     case TR_PARAM_LIVE_COLUMNS:
+    case TR_PARAM_SORT_TILE:
+    case TR_PARAM_SPREAD_SEGMENT:
       TR_FAIL(kInvalidArgument, "read-only plan parameter " + std::to_string(param));
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));
   }
@@ -949,6 +969,8 @@ int tr_plan_get_param(const tr_plan* p, int param, int64_t* value) {
     case TR_PARAM_SUPPORT_POINTS: *value = p->support_points; return kOk;
     case TR_PARAM_LIVE_COLUMNS: *value = p->live_cols; return kOk;
     case TR_PARAM_SPREAD_ORDER: *value = p->spread_order; return kOk;
+    case TR_PARAM_SORT_TILE: *value = p->tile; return kOk;
+    case TR_PARAM_SPREAD_SEGMENT: *value = p->seg; return kOk;
     default: TR_FAIL(kInvalidArgument, "unknown plan parameter " + std::to_string(param));
   }
 }
@@ -978,6 +1000,16 @@ int tr_plan_gather_weights(const tr_plan* p, int bin, const float* src, float* d
   const BinData& b = p->bins[bin];
   if (b.M == 0) return kOk;
   hipLaunchKernelGGL(k_gather_samples<float>, dim3(grid1d(b.M, 256)), dim3(256), 0,
+                     (hipStream_t)stream, src, b.perm, b.M, 1, dst);
+  TR_LAUNCH_CHECK();
+  return kOk;
+}
+
+int tr_plan_scatter_weights(const tr_plan* p, int bin, const float* src, float* dst, void* stream) {
+  if (int r = check_plan_bin(p, bin)) return r;
+  const BinData& b = p->bins[bin];
+  if (b.M == 0) return kOk;
+  hipLaunchKernelGGL(k_scatter_samples<float>, dim3(grid1d(b.M, 256)), dim3(256), 0,
                      (hipStream_t)stream, src, b.perm, b.M, 1, dst);
   TR_LAUNCH_CHECK();
   return kOk;

@@ -38,6 +38,8 @@ SIGNATURES = {
     "tr_clock_rate": (c_int, [ctypes.POINTER(c_i64)]),
     "tr_plan_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int,
                                ctypes.POINTER(c_vp), ctypes.POINTER(c_i64), c_vp]),
+    "tr_plan_create_geom": (c_int, [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int,
+                                    ctypes.POINTER(c_vp), ctypes.POINTER(c_i64), c_int, c_int, c_vp]),
     "tr_plan_destroy": (c_int, [c_vp]),
     "tr_plan_info": (c_int, [c_vp, c_int, ctypes.POINTER(c_i64)]),
     "tr_plan_release_workspace": (c_int, [c_vp]),
@@ -46,6 +48,7 @@ SIGNATURES = {
     "tr_plan_gather_samples": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp]),
     "tr_plan_scatter_samples": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp]),
     "tr_plan_gather_weights": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
+    "tr_plan_scatter_weights": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
     "tr_nufft_forward": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
     "tr_nufft_adjoint": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_int, c_vp]),
     "tr_nufft_forward_grid": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp]),

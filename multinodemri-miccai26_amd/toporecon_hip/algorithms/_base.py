@@ -259,14 +259,49 @@ class HipPdhgBase:
                 self.y[b].div_(np.float32(mx))
         return mx
 
+    def set_geometry(self, sort_tile: int = 8, spread_segment: int = 1024):
+        """Rebuild the NUFFT plan with another sample geometry (sort tile edge, heavy-cluster
+        segment length) and move the sorted per-sample data (k-space, data duals, DCF) to its
+        sample order; launch parameters carry over."""
+        import torch
+        from ..nufft_backend import NufftPlan
+        old = self.plan
+        if (old.get_param("sort_tile"), old.get_param("spread_segment")) == (int(sort_tile), int(spread_segment)):
+            return
+        new = NufftPlan(self.img_shape, old._coords, self.device, sort_tile=sort_tile,
+                        spread_segment=spread_segment)
+        for k in NufftPlan.LAUNCH_PARAMS:
+            new.set_param(k, old.get_param(k))
+        s = self.stream
+        with torch.cuda.device(self.device):
+            for b in range(self.B):
+                M = self.y[b].shape[0]
+                if M == 0:
+                    continue
+                orig = torch.empty((self.K, M), dtype=torch.complex64, device=self.device)
+                for t in (self.y[b], self.q[b], self.q0[b]):
+                    if t is None:
+                        continue
+                    _lib.call("tr_plan_scatter_samples", old.handle, b, t.data_ptr(), self.K, orig.data_ptr(), s)
+                    _lib.call("tr_plan_gather_samples", new.handle, b, orig.data_ptr(), self.K, t.data_ptr(), s)
+                d = torch.empty_like(self.dcf[b])
+                _lib.call("tr_plan_scatter_weights", old.handle, b, self.dcf[b].data_ptr(), d.data_ptr(), s)
+                _lib.call("tr_plan_gather_weights", new.handle, b, d.data_ptr(), self.dcf[b].data_ptr(), s)
+                del orig, d
+            torch.cuda.synchronize(self.device)
+        old.close()
+        self.plan = new
+
     def autotune(self, reps: int = 2, cache_path=None, force: bool = False, bin_streams: bool = True,
-                 workspace_budget: float = 64 * 2.0 ** 30) -> dict:
+                 workspace_budget: float = 64 * 2.0 ** 30, geometry: bool = False) -> dict:
         """Pick the plan's gridding launch parameters by timing them on the largest bin, then the
         number of bin streams (each with its own grid workspace, within workspace_budget bytes)
         by timing the adjoint data term.
 
         Only the workgroup order and stream assignment change, so run() gives bit-identical
-        results whatever is chosen.
+        results whatever is chosen.  With geometry=True the sample geometry (gridding sort tile
+        and heavy-cluster segment length) is tuned first on plans of the largest bin and the
+        solver's plan rebuilt with it (set_geometry); that choice changes float32 rounding.
         """
         import time as _time
 
@@ -282,6 +317,15 @@ class HipPdhgBase:
         out = _autotune.tune(self.plan, image=img, maps=self.maps, E=self.E, C=self.C, ksp=ksp,
                              dcf=self.dcf[b], stream=self.stream, bins=(b,), reps=reps,
                              cache_path=cache_path, force=force)
+        if geometry:
+            launch = {k: self.plan.get_param(k) for k in ("interp_xcd_chunk", "interp_kgroups",
+                                                          "spread_xcd_chunk")}
+            geo = _autotune.tune_geometry(self.img_shape, self.plan._coords[b], image=img, maps=self.maps,
+                                          E=self.E, C=self.C, dcf=self.dcf[b], device=self.device,
+                                          stream=self.stream, reps=reps, launch=launch)
+            self.set_geometry(geo["sort_tile"], geo["spread_segment"])
+            out = dict(out, sort_tile=geo["sort_tile"], spread_segment=geo["spread_segment"],
+                       timings_ms=dict(out.get("timings_ms", {}), geometry=geo["timings_ms"]))
         del img, ksp
         if bin_streams and self.B > 1:
             ws = float(np.prod(self.plan.grid_shape)) * self.K * 8.0

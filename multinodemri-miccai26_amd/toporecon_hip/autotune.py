@@ -21,6 +21,10 @@ import os
 from . import _lib
 
 INTERP_CANDIDATES = (0, 16, 64, 256, 1024)
+# sample geometry of a plan (fixed when it is built): sort tile edges, then heavy-cluster segment
+# lengths, tried one axis at a time from the defaults
+TILE_CANDIDATES = (4, 8, 16)
+SEGMENT_CANDIDATES = (256, 1024, 4096)
 # all channels per block / 64-channel groups / 16-byte loads over <=128-channel groups / the same with
 # one gather per group of samples with identical taps
 INTERP_KGROUPS = (0, 1, 2, 3)
@@ -106,3 +110,57 @@ def tune(plan, *, image, maps, E: int, C: int, ksp, dcf, stream, bins=(0,), reps
         with open(cache_path, "w") as f:
             json.dump(disk, f, indent=1, sort_keys=True)
     return dict(out, cached=False)
+
+
+def tune_geometry(img_shape, coords, *, image, maps, E: int, C: int, dcf, device, stream, reps: int = 2,
+                  launch: dict | None = None) -> dict:
+    """Pick the sample geometry (sort tile edge, then heavy-cluster segment length) for a plan of
+    ``coords`` (one bin's (M, 3) float32 device coordinates, the largest bin of the problem).
+
+    Every candidate is a plan of its own built from ``coords`` with the launch parameters in
+    ``launch`` (the tuned ones of the problem's plan); the interpolation and the spreading of the
+    data term are timed on it with HIP events and the geometry with the smallest sum is kept.
+    ``image`` (E, Z, Y, X) and ``dcf`` (M,) are scratch of the right sizes.  Unlike the launch
+    parameters, the geometry changes the order in which float32 products are added, so the
+    results of two geometries agree to rounding, not bit for bit.
+    Returns {"sort_tile", "spread_segment", "timings_ms"}.
+    """
+    import torch
+    from .nufft_backend import NufftPlan
+    K = E * C
+    M = int(coords.shape[0])
+    ksp = torch.zeros((M, K), dtype=torch.complex64, device=image.device)
+
+    def cost(tile, seg):
+        plan = NufftPlan(img_shape, [coords], device, sort_tile=tile, spread_segment=seg)
+        try:
+            for k, v in (launch or {}).items():
+                if k in NufftPlan.LAUNCH_PARAMS:
+                    plan.set_param(k, v)
+
+            def fwd():
+                _lib.call("tr_data_simulate", plan.handle, 0, image.data_ptr(), maps.data_ptr(), E, C,
+                          ksp.data_ptr(), stream)
+
+            def adj():
+                _lib.call("tr_data_adjoint", plan.handle, 0, ksp.data_ptr(), dcf.data_ptr(), maps.data_ptr(),
+                          E, C, image.data_ptr(), 0, stream)
+
+            fwd()
+            adj()  # warm both paths (workspace, partial sums) before timing
+            return _time_kernel("interp", fwd, reps) + _time_kernel("spread", adj, reps)
+        finally:
+            plan.close()
+
+    timings = {}
+    best_t, best_s = 8, 1024
+    for t in TILE_CANDIDATES:
+        timings[(t, best_s)] = cost(t, best_s)
+    best_t = min((k for k in timings), key=timings.get)[0]
+    for sg in SEGMENT_CANDIDATES:
+        if (best_t, sg) not in timings:
+            timings[(best_t, sg)] = cost(best_t, sg)
+    best_t, best_s = min(timings, key=timings.get)
+    del ksp
+    return {"sort_tile": best_t, "spread_segment": best_s,
+            "timings_ms": {f"{t}x{sg}": round(v, 4) for (t, sg), v in timings.items()}}

@@ -86,7 +86,7 @@ def resolve_device(device: Any):
 class NufftPlan:
     """Owner of a native ``tr_plan`` (coordinates, sort permutation, CSR, workspace)."""
 
-    def __init__(self, img_shape, coords_dev, device):
+    def __init__(self, img_shape, coords_dev, device, sort_tile=8, spread_segment=1024):
         import torch
         self.device = device
         self.img_shape = tuple(int(v) for v in img_shape)
@@ -98,8 +98,9 @@ class NufftPlan:
         lib = _lib.load()
         with torch.cuda.device(device):
             stream = _lib.stream_handle(device)
-            rc = lib.tr_plan_create(ctypes.byref(handle), *self.img_shape, n, arr_p, arr_m, stream)
-        _lib.check(rc, "tr_plan_create")
+            rc = lib.tr_plan_create_geom(ctypes.byref(handle), *self.img_shape, n, arr_p, arr_m, int(sort_tile),
+                                         int(spread_segment), stream)
+        _lib.check(rc, "tr_plan_create_geom")
         self.handle = handle
         self.num_bins = n
         info = self.info(0)
@@ -125,10 +126,13 @@ class NufftPlan:
 
     PARAMS = {"interp_xcd_chunk": 1, "spread_xcd_chunk": 2, "prune": 3, "support_points": 4,
               "live_columns": 5, "interp_kgroups": 6, "workspace_slots": 7, "lines_xcd_lines": 8,
-              "spread_order": 10}
+              "spread_order": 10, "sort_tile": 11, "spread_segment": 12}
+    LAUNCH_PARAMS = ("interp_xcd_chunk", "spread_xcd_chunk", "prune", "interp_kgroups", "workspace_slots",
+                     "lines_xcd_lines", "spread_order")
 
     def set_param(self, name: str, value: int):
-        """Set a launch parameter (block order only; results are unchanged)."""
+        """Set a launch parameter (block order only; results are unchanged).  sort_tile and
+        spread_segment are fixed when the plan is built (constructor arguments)."""
         _lib.call("tr_plan_set_param", self.handle, self.PARAMS[name], int(value))
 
     def get_param(self, name: str) -> int:

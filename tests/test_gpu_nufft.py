@@ -296,6 +296,62 @@ def test_spreading_orders_and_heavy_segments(torch_mod):
     plan.close()
 
 
+@pytest.mark.parametrize("tile,seg", [(4, 1024), (16, 1024), (32, 64), (8, 64), (2, 65536)])
+def test_sample_geometry_forward_adjoint(torch_mod, tile, seg):
+    """Plans built with other sort tiles and heavy-cluster segment lengths (64: the centre
+    clusters of a cone trajectory are cut into dozens of segments; 65536: never cut) match the
+    oracle forward and density-weighted adjoint, rerun bit for bit, and report their geometry."""
+    torch = torch_mod
+    from toporecon_hip import _lib, synth
+    from toporecon_hip.nufft_backend import NufftPlan
+    print("geometry", tile, seg, flush=True)
+    shape = (12, 16, 16)
+    coord = synth.cone_trajectory(2000, 24, shape).reshape(-1, 3)
+    dev = torch.device("cuda", 0)
+    plan = NufftPlan(shape, [torch.from_numpy(coord).to(dev)], dev, sort_tile=tile, spread_segment=seg)
+    assert (plan.get_param("sort_tile"), plan.get_param("spread_segment")) == (tile, seg)
+    with pytest.raises(Exception):
+        plan.set_param("sort_tile", 8)
+    E, C = 2, 3
+    K = E * C
+    M = plan.samples(0)
+    rng = np.random.default_rng(tile * 7 + seg)
+    img = _cplx(rng, (E, *shape))
+    maps = _cplx(rng, (C, *shape))
+    q = _cplx(rng, (K, M))
+    dcf = rng.uniform(0.2, 1.0, M).astype(np.float32)
+    it, mt, qt, dt = (torch.from_numpy(v).to(dev) for v in (img, maps, q, dcf))
+    st = _lib.stream_handle(dev)
+    fwd = torch.empty((K, M), dtype=torch.complex64, device=dev)
+    _lib.call("tr_nufft_forward", plan.handle, 0, it.data_ptr(), mt.data_ptr(), E, C, fwd.data_ptr(), st)
+    outs = []
+    for _ in range(2):
+        g = torch.empty((E, *shape), dtype=torch.complex64, device=dev)
+        _lib.call("tr_nufft_adjoint", plan.handle, 0, qt.data_ptr(), dt.data_ptr(), mt.data_ptr(), E, C,
+                  g.data_ptr(), 0, st)
+        outs.append(g.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
+    f = fwd.cpu().numpy()
+    mp = maps.astype(np.complex128)
+    for e in range(E):
+        for c in range(C):
+            ref = on.nufft(img[e].astype(np.complex128) * mp[c], coord)
+            assert _rel(f[e * C + c], ref) < TOL
+        ref = sum(np.conj(mp[c]) * on.nufft_adjoint(q[e * C + c] * dcf, coord, shape) for c in range(C))
+        assert _rel(outs[0][e], ref) < TOL
+    plan.close()
+
+
+def test_sample_geometry_limits(torch_mod):
+    torch = torch_mod
+    from toporecon_hip.nufft_backend import NufftPlan
+    dev = torch.device("cuda", 0)
+    c = [torch.zeros((10, 3), dtype=torch.float32, device=dev)]
+    for tile, seg in ((1, 1024), (33, 1024), (8, 63), (8, 65537)):
+        with pytest.raises(Exception):
+            NufftPlan((8, 8, 8), c, dev, sort_tile=tile, spread_segment=seg)
+
+
 @pytest.mark.parametrize("shape", [(13, 9, 7), (136, 10, 8), (8, 150, 9), (9, 8, 220), (6, 256, 7)])
 def test_general_grid_lengths(torch_mod, shape):
     """Grid lengths the compile-time FFT plans do not cover (17, 170, 188, 275 with prime factors

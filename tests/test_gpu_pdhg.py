@@ -281,3 +281,25 @@ def test_steps_beyond_the_bound_run_to_the_cap_like_the_oracle(torch_mod):
     assert np.all(np.isfinite(ubar))
     np.testing.assert_allclose(s.history, hist, rtol=1e-3)
     assert rel(ubar, o_ubar) < 1e-4
+
+
+def test_sample_geometry_rebuild_keeps_the_solution(torch_mod):
+    """A solver whose plan is rebuilt with another sample geometry (set_geometry moves k-space,
+    data duals and DCF to the new sample order) follows the oracle iterates like the default
+    plan; the geometry autotuner picks one of its candidates and keeps the solver on the oracle."""
+    from toporecon_hip import autotune
+    print("geometry rebuild", flush=True)
+    acq, ksp, prob = make_problem(seed=3, algorithm="tvme", l2=True)
+    o_ubar, _, st, _ = _oracle_run(prob, "tvme", 5, True)
+    s = _hip_solver("tvme", acq, ksp, 5, 0.0, True)
+    s.set_geometry(4, 64)
+    assert (s.plan.get_param("sort_tile"), s.plan.get_param("spread_segment")) == (4, 64)
+    ubar = s.run().cpu().numpy()
+    assert rel(ubar, o_ubar) < 2e-5
+    assert rel(s.u.cpu().numpy(), st.u) < 2e-5
+    s2 = _hip_solver("tvme", acq, ksp, 5, 0.0, True)
+    tuned = s2.autotune(reps=1, bin_streams=False, geometry=True)
+    assert tuned["sort_tile"] in autotune.TILE_CANDIDATES
+    assert tuned["spread_segment"] in autotune.SEGMENT_CANDIDATES
+    assert s2.plan.get_param("sort_tile") == tuned["sort_tile"]
+    assert rel(s2.run().cpu().numpy(), o_ubar) < 2e-5
