@@ -102,6 +102,9 @@ def parse_args(argv=None):
     ap.add_argument("--variants", default="",
                     help="comma list of extra warm starts 'cnn+K' / 'zero+K' (K CG steps) timed once each "
                          "and reported beside the learned leg, e.g. cnn+0,zero+4,cnn+3,cnn+5")
+    ap.add_argument("--geometry", default="default",
+                    help="sample geometry of the plan: 'default' (sort tile 8, segments of 1024 pairs), "
+                         "'tune' (the autotuner picks both) or 'TILExSEGMENT', e.g. 4x256")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the solve, copy a buffer of this many MiB once (a kernel of known "
                          "traffic for scaling hardware byte counters)")
@@ -439,12 +442,14 @@ def bench_batched(args, verbose=True):
                              margin=0, device=dev, bin_edges=acq.bin_edges, show_pbar=False,
                              echoes=len(cfg.echo_times_ms))
         s.simulate_kspace(acq.truth, cfg.snr_db, seed=cfg.seed + 104729)
-        tuned = s.autotune(reps=1, bin_streams=False)  # graph batches keep one stream per instance
+        tuned = s.autotune(reps=1, bin_streams=False, geometry=True)  # batches keep one stream per instance
         solvers.append(s)
         rows.append({"seed": cfg.seed, "shape": list(cfg.img_shape), "coils": cfg.coils,
                      "echoes": len(cfg.echo_times_ms), "bins": cfg.bins,
                      "samples_per_echo": int(acq.coord.shape[0] * acq.coord.shape[1]),
-                     "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "spread_xcd_chunk")}})
+                     "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "interp_kgroups",
+                                                                   "spread_xcd_chunk", "sort_tile",
+                                                                   "spread_segment")}})
         log(f"instance seed {cfg.seed}: {cfg.img_shape} C={cfg.coils} E={len(cfg.echo_times_ms)} B={cfg.bins} "
             f"samples/echo={rows[-1]['samples_per_echo']} ({time.perf_counter() - t0:.0f}s)", verbose)
     for s in solvers:  # warm every code path once
@@ -465,7 +470,8 @@ def bench_batched(args, verbose=True):
     # batched: one HIP-graph launch per iteration for every running instance, and the same
     # iteration issued eagerly with one stream per instance
     legs = {}
-    for mode in ("graph", "streams"):
+    modes = ("persistent", "graph", "streams")
+    for mode in modes:
         batch = GraphBatch(solvers)
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
@@ -477,10 +483,14 @@ def bench_batched(args, verbose=True):
             r[mode] = {"iterations": o["iterations"], "seconds": o["seconds"]}
         legs[mode] = {"seconds": tb, "captures": batch.captures, "bit_identical_to_sequential": same,
                       "iterations": sum(o["iterations"] for o in out)}
+        if mode == "persistent":
+            legs[mode]["replays"] = batch.replays
+            legs[mode]["timing"] = ("wall clock around run() (prepare, one capture, replays); per-instance "
+                                    "seconds from device clock stamps when each instance's gate closed")
         log(f"{mode}: {tb:.2f} s ({batch.captures} graph captures), identical results: {same}", verbose)
     its = sum(r["sequential"]["iterations"] for r in rows)
     for r in rows:
-        for leg in ("sequential", "graph", "streams"):
+        for leg in ("sequential",) + modes:
             r[leg]["iterations_per_s"] = r[leg]["iterations"] / max(r[leg]["seconds"], 1e-9)
     best = min(legs, key=lambda m: legs[m]["seconds"])
     t_batch = legs[best]["seconds"]
@@ -1025,9 +1035,14 @@ def bench_full(args, verbose=True):
         tuned = _set_params(solver, args.params, verbose)
     else:
         # bins stay on one stream: the per-kernel HIP events then time one kernel each
-        tuned = solver.autotune(bin_streams=False)
+        if args.geometry not in ("default", "tune"):
+            tile, seg = (int(v) for v in args.geometry.lower().split("x"))
+            solver.set_geometry(tile, seg)
+        tuned = solver.autotune(bin_streams=False, geometry=args.geometry == "tune")
         log(f"autotune in {time.perf_counter() - t0:.1f} s: "
-            f"{ {k: tuned.get(k) for k in ('interp_xcd_chunk', 'interp_kgroups', 'spread_xcd_chunk')} }", verbose)
+            f"{ {k: tuned.get(k) for k in ('interp_xcd_chunk', 'interp_kgroups', 'spread_xcd_chunk')} } "
+            f"geometry {solver.plan.get_param('sort_tile')}x{solver.plan.get_param('spread_segment')} "
+            f"{tuned.get('timings_ms', {}).get('geometry', '')}", verbose)
 
     from toporecon_hip.learned import DEFAULT_CG_STEPS, WarmStartNet, default_weights_path, warm_start
     cg_steps = DEFAULT_CG_STEPS if args.cg_steps is None else args.cg_steps
@@ -1238,6 +1253,9 @@ def bench_full(args, verbose=True):
                    "samples_per_bin": samples, "grid": list(solver.plan.grid_shape),
                    "launch_params": {k: tuned.get(k) for k in ("interp_xcd_chunk", "interp_kgroups",
                                                                 "spread_xcd_chunk")},
+                   "geometry": {"sort_tile": solver.plan.get_param("sort_tile"),
+                                "spread_segment": solver.plan.get_param("spread_segment"),
+                                "mode": args.geometry},
                    "bin_streams": 1,
                    "final_tol": solver.history[-1] if solver.history else None,
                    "run_wall_s": time.perf_counter() - T_PROCESS0},

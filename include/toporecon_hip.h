@@ -328,6 +328,19 @@ int tr_js_rss_normalize(tr_c64* maps, int C, int64_t N, void* stream);
 /* y += a x (n complex values): the lamda * I term of JsenseRecon's regularised least squares. */
 int tr_js_axpy(tr_c64* y, const tr_c64* x, int64_t n, float a, void* stream);
 
+/* ---- Respiratory self-gating (src/toporecon/preprocessing/respiratory.py:86-92): the band-pass of
+ * every (echo, coil) centre trace.  traces (ntraces, T) float32 (magnitude minus its mean, computed
+ * by the caller as the reference does), out (ntraces, T) float32 = sosfiltfilt(hi, sosfiltfilt(lo,
+ * trace)) with SciPy's semantics: odd extension of pad_* samples at each end (in the input's
+ * precision: float32 for the first filter, float64 for the second), a forward and a backward pass of
+ * the n_* second-order sections sos_* (n, 6) started from zi_* (n, 2) times the pass's first sample,
+ * float64 arithmetic with every product and sum rounded in the order of SciPy's loop.  n_* <= 16,
+ * pad_* < T.  work holds tr_resp_work_count() float64 values.  One thread per trace. */
+int64_t tr_resp_work_count(int64_t ntraces, int64_t T, int pad_lo, int pad_hi);
+int tr_resp_bandpass(const float* traces, int64_t ntraces, int64_t T, const double* sos_lo, const double* zi_lo,
+                     int n_lo, int pad_lo, const double* sos_hi, const double* zi_hi, int n_hi, int pad_hi,
+                     double* work, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

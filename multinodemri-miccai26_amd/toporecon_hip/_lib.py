@@ -94,6 +94,9 @@ SIGNATURES = {
     "tr_js_axis_apply": (c_int, [c_vp, c_i64, c_int, c_i64, c_vp, c_int, c_vp, c_vp]),
     "tr_js_rss_normalize": (c_int, [c_vp, c_int, c_i64, c_vp]),
     "tr_js_axpy": (c_int, [c_vp, c_vp, c_i64, c_f32, c_vp]),
+    "tr_resp_work_count": (c_i64, [c_i64, c_i64, c_int, c_int]),
+    "tr_resp_bandpass": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp,
+                                 c_vp]),
 }
 
 _lib = None

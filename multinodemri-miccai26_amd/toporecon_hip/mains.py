@@ -264,7 +264,7 @@ def tvmw_main(argv=None) -> int:
 
 
 def cli_main(argv=None) -> int:
-    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS``, ``resp INPUT --work-dir DIR`` and ``jsense
+    """``reconstruct --algorithm {tvm,tvme,tvmw} [--] ARGS``, ``resp INPUT --work-dir DIR [--device]`` and ``jsense
     INPUT --work-dir DIR [--device --readout-fraction --coordinate-scaling --fov-scale --show-progress]``
     as in src/toporecon/cli.py:23-57, :75-88, :135-196.  The shards are the reference's format, so its own ``toporecon
     stitch`` assembles them."""
@@ -277,6 +277,8 @@ def cli_main(argv=None) -> int:
     rp = sub.add_parser("resp", help="Estimate resp.hdr/.cfl in a separate work directory.")
     rp.add_argument("input", type=Path)
     rp.add_argument("--work-dir", type=Path, required=True)
+    rp.add_argument("--device", type=int, default=None,
+                    help="GPU index for the band-pass of the centre traces (default: SciPy on the host).")
     jp = sub.add_parser("jsense", help="Estimate mps.hdr/.cfl in a separate work directory.")
     jp.add_argument("input", type=Path)
     jp.add_argument("--work-dir", type=Path, required=True)
@@ -294,7 +296,7 @@ def cli_main(argv=None) -> int:
     if args.command == "resp":
         from .preprocessing import estimate_respiratory_from_dataset
         args.work_dir.mkdir(parents=True, exist_ok=True)
-        out = estimate_respiratory_from_dataset(args.input, args.work_dir)
+        out = estimate_respiratory_from_dataset(args.input, args.work_dir, device=args.device)
         print(f"Wrote {out}.hdr and {out}.cfl")
         return 0
     if args.command == "jsense":
