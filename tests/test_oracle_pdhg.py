@@ -136,3 +136,15 @@ def test_stopping_rules():
     assert op.rel_change_tvm(u0, u0) == 0.0
     assert op.rel_change_tvm(np.zeros_like(u0), u0) == float("inf")
     assert op.rel_change_tvmw(np.zeros_like(u0), u0) == pytest.approx(np.sqrt(8) / 1e-12)
+
+
+def test_run_resumes_from_a_state():
+    """run(init=state) continues a solve: 1 + 2 iterations equal 3 in one go, bit for bit."""
+    from problems import make_problem
+    _, _, prob = make_problem(shape=(6, 8, 8), coils=2, echoes=2, bins=2, interleaves=30, readouts=16, seed=5,
+                              algorithm="tvmw")
+    kw = dict(lam1=2e-3, lam2=2e-3, lam3=2e-3, l2_coupling=True, tol=0.0)
+    u3, n3, _ = op.run(prob, "tvmw", max_iter=3, **kw)
+    _, _, st = op.run(prob, "tvmw", max_iter=1, **kw)
+    u12, _, _ = op.run(prob, "tvmw", max_iter=2, init=st, **kw)
+    assert n3 == 3 and np.array_equal(u3, u12)
