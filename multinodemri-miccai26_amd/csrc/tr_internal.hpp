@@ -324,6 +324,20 @@ struct SpreadArgs {
   const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };
 
+// Raw buffer accesses with a lane offset at or past the record count read
+// zero and write nothing, so masked lanes need no per-lane branch and make no
+// memory request; a wave with no live lane skips the instruction altogether
+// (one scalar branch), which matters where support pruning masks whole waves.
+// The base is a uniform pointer (scalar registers); each lane adds one 32-bit
+// byte offset, and a per-instruction scalar offset can step between rows.
+constexpr int kBufRecords = 0x7fffffff;
+constexpr unsigned kBufOob = 0x80000000u;
+constexpr int kBufWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(char* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(p, 0, kBufRecords, kBufWord3);
+}
+
 // Position p of an axis of length g lies in the window (start, len) of that axis.
 __device__ __forceinline__ bool in_window(int p, int2 w, int g) {
   int d = p - w.x;

@@ -21,18 +21,6 @@ __device__ __forceinline__ bool arc_has(int p, int start, int len) {
   return min(d, d + (unsigned)L) < (unsigned)len;
 }
 
-// Raw buffer accesses with a lane offset at or past the record count read
-// zero and write nothing, so masked lanes need no per-lane branch and make no
-// memory request; a wave with no live lane skips the instruction altogether
-// (one scalar branch), which matters where support pruning masks whole waves.
-constexpr int kBufRecords = 0x7fffffff;
-constexpr unsigned kBufOob = 0x80000000u;
-constexpr int kBufWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(char* p) {
-  return __builtin_amdgcn_make_buffer_rsrc(p, 0, kBufRecords, kBufWord3);
-}
-
 // Two-level register FFT of one line tile: thread (ch, w) loads positions
 // w + N2 r, runs pass 1 into the LDS tile, and after the barrier pass 2 gives
 // it positions w + N1 k2. Every position row r is a uniform base pointer (blk

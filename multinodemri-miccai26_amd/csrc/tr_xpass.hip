@@ -164,6 +164,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
     // LDS by image column, where thread x adds the chunk's coils in increasing order (the same
     // products in the same order as the stage-by-stage path)
     constexpr int N1 = split_n1(LC), N2 = LC / N1;
+    // The block's rounds (coil chunk, echo) run in order, and each round's 16 grid loads are issued
+    // one round ahead, so they are in flight while the previous round is transformed.  The grid row
+    // is a raw buffer: each lane adds one 32-bit byte offset and the position row r of a load is a
+    // scalar offset (r N2 K 8 bytes); workers past N2 get an offset past the record count and read 0
+    // without touching memory.
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(const_cast<char*>(reinterpret_cast<const char*>(src)));
+    const unsigned rowb = (unsigned)N2 * (unsigned)a.K * 8u;
+    auto load_round = [&](int c0r, int er, float2 (&dst)[N1]) {
+      const int chr = c0r + min(ch, min(KC, a.C - c0r) - 1);
+      const unsigned lo = w < N2 ? ((unsigned)w * (unsigned)a.K + (unsigned)(er * a.C + chr)) * 8u : kBufOob;
+#pragma unroll
+      for (int r = 0; r < N1; ++r)
+        dst[r] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, lo, (int)(r * rowb), 0));
+    };
+    float2 vn[N1];
+    load_round(0, 0, vn);
     float2 acc[kMaxE];
 #pragma unroll
     for (int e = 0; e < kMaxE; ++e) acc[e] = make_float2(0.f, 0.f);
@@ -188,9 +204,14 @@ __global__ void __launch_bounds__(kBlock) k_adj_x_crop(AdjXArgs a) {
           float2 v[N1];
           const bool lv = w < N2 && ch < cn;
 #pragma unroll
-          for (int r = 0; r < N1; ++r) v[r] = src[(int64_t)min(w + N2 * r, LC - 1) * a.K + (int64_t)e * a.C + chc];
+          for (int r = 0; r < N1; ++r) v[r] = vn[r];
 #pragma unroll
-          for (int r = 0; r < N1; ++r) keep_loaded(v[r]);
+          for (int r = 0; r < N1; ++r) keep_loaded(v[r]);  // this round's loads have landed
+          if (e + 1 < E)
+            load_round(c0, e + 1, vn);
+          else if (c0 + KC < a.C)
+            load_round(c0 + KC, 0, vn);
+          __builtin_amdgcn_sched_barrier(0);  // the next round's loads stay ahead of this round's work
 #pragma unroll
           for (int r = 0; r < N1; ++r) v[r] = lv ? cconj(v[r]) : make_float2(0.f, 0.f);
           __syncthreads();  // twiddles / the previous accumulation's reads
