@@ -38,60 +38,76 @@ struct AxisArgs {
   const int* gate = nullptr;  // batched solves: the launch returns at once while *gate != 0
 };
 
-// Analysis along one axis: lo[k] = sum_j h[j] x[2k + 1 - j], zeros outside [0, n).
+// Analysis along one axis: lo[k] = sum_j h[j] x[2k + 1 - j], zeros outside [0, n).  One thread
+// computes the coefficient pair k = 2p, 2p + 1, whose taps x[4p - 10 .. 4p + 3] overlap in 10 of 12:
+// 14 loads for two coefficients instead of 24.  Every tap is loaded from a clamped, in-bounds index
+// before any is used (no load sits behind a per-lane branch); taps outside [0, n) are then skipped,
+// and each sum runs over j in increasing order as before.
 __global__ void k_dwt_axis(AxisArgs a, const float2* __restrict__ x, float2* __restrict__ lo,
                            float2* __restrict__ hi) {
   if (gate_closed(a.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = a.rows * a.c * a.Q;
+  const int cp = (a.c + 1) / 2;  // coefficient pairs per row
+  const int64_t total = a.rows * cp * a.Q;
   if (t >= total) return;
   const int64_t q = t % a.Q;
-  const int k = (int)((t / a.Q) % a.c);
-  const int64_t row = t / (a.Q * a.c);
+  const int p = (int)((t / a.Q) % cp);
+  const int64_t row = t / (a.Q * cp);
   const int64_t img = row / a.rows_img;
   const int64_t r = row - img * a.rows_img;
   const float2* xs = x + img * a.sig_stride + r * a.n * a.Q + q;
-  // every tap is loaded from a clamped, in-bounds index before any is used (no load sits behind a
-  // per-lane branch); taps outside [0, n) are then skipped as before
-  float2 v[kFilt];
+  const int m0 = 4 * p - 10;  // first tap of the pair
+  float2 v[kFilt + 2];
 #pragma unroll
-  for (int j = 0; j < kFilt; ++j) v[j] = xs[(int64_t)min(max(2 * k + 1 - j, 0), a.n - 1) * a.Q];
-  float2 sl = make_float2(0.f, 0.f), sh = sl;
+  for (int i = 0; i < kFilt + 2; ++i) v[i] = xs[(int64_t)min(max(m0 + i, 0), a.n - 1) * a.Q];
+  const int k0 = 2 * p, k1 = 2 * p + 1;
+  float2 sl0 = make_float2(0.f, 0.f), sh0 = sl0, sl1 = sl0, sh1 = sl0;
 #pragma unroll
   for (int j = 0; j < kFilt; ++j) {
-    const int m = 2 * k + 1 - j;
-    if (m >= 0 && m < a.n) {
-      sl.x += c_dec_lo[j] * v[j].x;
-      sl.y += c_dec_lo[j] * v[j].y;
-      sh.x += c_dec_hi[j] * v[j].x;
-      sh.y += c_dec_hi[j] * v[j].y;
+    const int ma = 2 * k0 + 1 - j, mb = 2 * k1 + 1 - j;  // v[11 - j], v[13 - j]
+    if (ma >= 0 && ma < a.n) {
+      sl0.x += c_dec_lo[j] * v[11 - j].x;
+      sl0.y += c_dec_lo[j] * v[11 - j].y;
+      sh0.x += c_dec_hi[j] * v[11 - j].x;
+      sh0.y += c_dec_hi[j] * v[11 - j].y;
+    }
+    if (mb >= 0 && mb < a.n) {
+      sl1.x += c_dec_lo[j] * v[13 - j].x;
+      sl1.y += c_dec_lo[j] * v[13 - j].y;
+      sh1.x += c_dec_hi[j] * v[13 - j].x;
+      sh1.y += c_dec_hi[j] * v[13 - j].y;
     }
   }
-  const int64_t o = (r * a.c + k) * a.Q + q;
-  lo[img * a.lo_stride + a.lo_off + o] = sl;
-  hi[img * a.hi_stride + a.hi_off + o] = sh;
+  const int64_t o = (r * a.c + k0) * a.Q + q;
+  lo[img * a.lo_stride + a.lo_off + o] = sl0;
+  hi[img * a.hi_stride + a.hi_off + o] = sh0;
+  if (k1 < a.c) {
+    lo[img * a.lo_stride + a.lo_off + o + a.Q] = sl1;
+    hi[img * a.hi_stride + a.hi_off + o + a.Q] = sh1;
+  }
 }
 
 // Synthesis along one axis (transpose of k_dwt_axis):
 // x[m] = sum_k lo[k] h_lo[2k + 1 - m] + hi[k] h_hi[2k + 1 - m].
+// The taps of x[m] are j = 2k + 1 - m in [0, 11], i.e. k = m / 2 + i, i = 0 .. 5 (j = 2i + 1 for
+// even m, 2i for odd m), k <= c - 1: the pair x[2p], x[2p + 1] reads the same 6 + 6 coefficients, so
+// one thread computes both from one set of loads.  All 12 values are loaded (clamped, in bounds)
+// before any is used; each sum runs over increasing k as before.
 __global__ void k_idwt_axis(AxisArgs a, const float2* __restrict__ lo, const float2* __restrict__ hi,
                             float2* __restrict__ x) {
   if (gate_closed(a.gate)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = a.rows * a.n * a.Q;
+  const int np = (a.n + 1) / 2;  // output pairs per row
+  const int64_t total = a.rows * np * a.Q;
   if (t >= total) return;
   const int64_t q = t % a.Q;
-  const int m = (int)((t / a.Q) % a.n);
-  const int64_t row = t / (a.Q * a.n);
+  const int p = (int)((t / a.Q) % np);
+  const int64_t row = t / (a.Q * np);
   const int64_t img = row / a.rows_img;
   const int64_t r = row - img * a.rows_img;
   const float2* ls = lo + img * a.lo_stride + a.lo_off + r * a.c * a.Q + q;
   const float2* hs = hi + img * a.hi_stride + a.hi_off + r * a.c * a.Q + q;
-  // taps j = 2k + 1 - m in [0, 11]  <=>  k = m / 2 + i, i = 0 .. 5 (j = 2i + 1 for even m, 2i for
-  // odd m), and k <= c - 1.  All 12 values are loaded (clamped, in bounds) before any is used; the
-  // sum runs over increasing k as before.
-  const int kb = m / 2;
-  const bool odd = (m & 1) != 0;
+  const int kb = p;
   float2 vl[6], vh[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
@@ -99,17 +115,19 @@ __global__ void k_idwt_axis(AxisArgs a, const float2* __restrict__ lo, const flo
     vl[i] = ls[kc * a.Q];
     vh[i] = hs[kc * a.Q];
   }
-  float2 s = make_float2(0.f, 0.f);
+  float2 se = make_float2(0.f, 0.f), so = se;
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     if (kb + i <= a.c - 1) {
-      const float fl = odd ? c_dec_lo[2 * i] : c_dec_lo[2 * i + 1];
-      const float fh = odd ? c_dec_hi[2 * i] : c_dec_hi[2 * i + 1];
-      s.x += fl * vl[i].x + fh * vh[i].x;
-      s.y += fl * vl[i].y + fh * vh[i].y;
+      se.x += c_dec_lo[2 * i + 1] * vl[i].x + c_dec_hi[2 * i + 1] * vh[i].x;
+      se.y += c_dec_lo[2 * i + 1] * vl[i].y + c_dec_hi[2 * i + 1] * vh[i].y;
+      so.x += c_dec_lo[2 * i] * vl[i].x + c_dec_hi[2 * i] * vh[i].x;
+      so.y += c_dec_lo[2 * i] * vl[i].y + c_dec_hi[2 * i] * vh[i].y;
     }
   }
-  x[img * a.sig_stride + (r * a.n + m) * a.Q + q] = s;
+  float2* xo = x + img * a.sig_stride + (r * a.n + 2 * p) * a.Q + q;
+  xo[0] = se;
+  if (2 * p + 1 < a.n) xo[a.Q] = so;
 }
 
 // p <- proj(p + sigma w): t * min(1, lam / sqrt(re^2 + im^2 + 1e-12)); 0 if lam <= 0
@@ -178,7 +196,7 @@ static int make_levels(int Z, int Y, int X, int level, std::vector<Level>& lv, i
 static int launch_axis(const AxisArgs& a0, const float2* x, float2* lo, float2* hi, hipStream_t s) {
   AxisArgs a = a0;
   a.gate = gate_current();
-  const int64_t total = a.rows * a.c * a.Q;
+  const int64_t total = a.rows * ((a.c + 1) / 2) * a.Q;  // one thread per coefficient pair
   if (total == 0) return kOk;
   hipLaunchKernelGGL(k_dwt_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, x, lo, hi);
   TR_LAUNCH_CHECK();
@@ -188,7 +206,7 @@ static int launch_axis(const AxisArgs& a0, const float2* x, float2* lo, float2* 
 static int launch_iaxis(const AxisArgs& a0, const float2* lo, const float2* hi, float2* x, hipStream_t s) {
   AxisArgs a = a0;
   a.gate = gate_current();
-  const int64_t total = a.rows * a.n * a.Q;
+  const int64_t total = a.rows * ((a.n + 1) / 2) * a.Q;  // one thread per output pair
   if (total == 0) return kOk;
   hipLaunchKernelGGL(k_idwt_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, lo, hi, x);
   TR_LAUNCH_CHECK();
