@@ -257,13 +257,7 @@ __global__ void __launch_bounds__(256) k_interp_grp(InterpArgs a, int G) {
   float4 acc[kTapGroup];
 #pragma unroll
   for (int t = 0; t < kTapGroup; ++t) acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-#ifndef TR_INTERP_ROWS
-#define TR_INTERP_ROWS 1
-#endif
-  // y rows whose 4 x taps are loaded together: one row keeps the kernel at 50 VGPRs (8 waves per
-  // SIMD), which hides the gathers' latency better than 16 loads in flight per wave at 94 VGPRs
-  // (5 waves): 2.80 -> 2.71 ms per launch at 192 channels
-  constexpr int RB = TR_INTERP_ROWS;
+  constexpr int RB = kInterpRows;
   for (int dz = 0; dz < nz; ++dz) {
     const int pz = wrap2(p0z + dz, gz);
 #pragma unroll
