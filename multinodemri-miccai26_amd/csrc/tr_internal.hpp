@@ -176,6 +176,7 @@ struct FwdXArgs {
   int C;
   int K;  // grid channel count (stride of a grid point's channel record)
   int E;  // echoes of this launch: img holds E images, grid channels e * C + c
+  int eg;  // echoes whose image rows the block stages in LDS together (1 .. E)
   float2* grid;
   const int* gate = nullptr;  // batched solves: skip the launch while *gate != 0
 };

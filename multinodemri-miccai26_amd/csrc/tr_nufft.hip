@@ -37,17 +37,17 @@ static int forward_to_grid(tr_plan* p, const float2* img, const float2* maps, in
     const int kc = fft_kc(C, p->ax[2].M);
     dim3 grid((unsigned)(nz * ny * ((C + kc - 1) / kc)));
     ProfScope ps(kKFwdXPad, 8.0 * (E + C) * p->N + 8.0 * K * nz * ny * gx, s);
-    // LDS: twiddles + FFT tile + the image rows of the launch's echoes (staged once for the
-    // block's coils); echoes go in groups that fit
+    // LDS: twiddles + FFT tile + the image rows of a group of echoes (staged for the block's coils,
+    // the coils' maps staying in registers across all echoes); the group is sized so that four
+    // blocks share a CU's 160 KiB (four waves per SIMD), else as many rows as 64 KiB holds
     const size_t tile = fft_smem(p->ax[2].M, kc), row = sizeof(float2) * (size_t)p->n[2];
     if (tile + row > 65536) TR_FAIL(kUnsupported, "image rows too long for the forward x pass (LDS)");
-    const int eg = (int)std::min<size_t>((size_t)E, (65536 - tile) / row);
-    for (int e0 = 0; e0 < E; e0 += eg) {
-      a.E = std::min(eg, E - e0);
-      a.img = img + (int64_t)e0 * p->N;
-      a.grid = p->work + (int64_t)e0 * C;
-      if (int r = launch_fwd_x(kc, gx, grid, tile + row * a.E, s, a)) return r;
-    }
+    const size_t budget = tile + row <= 40960 ? 40960 : 65536;
+    a.E = E;
+    a.eg = (int)std::max<size_t>(1, std::min<size_t>((size_t)E, (budget - tile) / row));
+    a.img = img;
+    a.grid = p->work;
+    if (int r = launch_fwd_x(kc, gx, grid, tile + row * a.eg, s, a)) return r;
   }
   {  // y pass over the nz occupied z-planes, windowed load along y
     const int kc = fft_kc(K, p->ax[1].M);

@@ -11,8 +11,16 @@ namespace tr {
 // Block = (coil chunk, image row); the block walks all echoes, so the coil
 // maps of the row are read once.
 // ---------------------------------------------------------------------------
+// Two-level lengths ask for 4 blocks per CU (<= 128 VGPRs, 4 waves per SIMD, with the image rows of
+// 4 echoes staged at a time so the LDS holds 4 blocks as well); the runtime-length path keeps its
+// registers.
 template <int KC, int LC>
-__global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
+constexpr int fwd_x_min_blocks() {
+  return (LC > 0 && two_level_ok<KC, LC>()) ? 4 : 1;
+}
+
+template <int KC, int LC>
+__global__ void __launch_bounds__(kBlock, (fwd_x_min_blocks<KC, LC>())) k_fwd_x_pad(FwdXArgs a) {
   if (gate_closed(a.gate)) return;
   extern __shared__ float2 smem[];
   using T = Tile<KC, LC>;
@@ -42,11 +50,16 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
     // select afterwards, so no load sits in a per-lane branch and all of them are in flight at once.
     // The E image rows (shared by the block's 16 coils) are staged in LDS behind the FFT tile.
     const int E = a.E;
+    const int EG = a.eg;  // echoes whose image rows are staged together
     float2* img_row = buf + LC * T::ld;
-    for (int i = tid; i < E * nx; i += kBlock) {
-      const int e = i / nx;
-      img_row[i] = a.img[(int64_t)e * N + rowvox + (i - e * nx)];
-    }
+    auto stage_rows = [&](int e0) {
+      const int ne = min(EG, E - e0);
+      for (int i = tid; i < ne * nx; i += kBlock) {
+        const int e = i / nx;
+        img_row[i] = a.img[(int64_t)(e0 + e) * N + rowvox + (i - e * nx)];
+      }
+    };
+    stage_rows(0);
     const int cc = min(c, a.C - 1);
     float2 sm[N1];
     float ap[N1];
@@ -72,20 +85,34 @@ __global__ void __launch_bounds__(kBlock) k_fwd_x_pad(FwdXArgs a) {
     const int zp = pmod(z - nz / 2, gz), yp = pmod(y - ny / 2, gy);
     __syncthreads();  // image rows staged
     for (int e = 0; e < E; ++e) {
+      if (e > 0 && e % EG == 0) {
+        // the next group's rows: every read of the previous group's rows lies before the barrier
+        // that preceded the previous echo's pass 1, which every thread has passed
+        stage_rows(e);
+        __syncthreads();
+      }
+      const int el = e % EG;
       float2 v[N1];
       // sm is zero at padding positions, so the product is the zero-padded, coil-weighted row
 #pragma unroll
       for (int r = 0; r < N1; ++r)
-        v[r] = cmul(img_row[e * nx + min(pmod(w + N2 * r + nx / 2, gx), nx - 1)], sm[r]);
+        v[r] = cmul(img_row[el * nx + min(pmod(w + N2 * r + nx / 2, gx), nx - 1)], sm[r]);
       __syncthreads();  // twiddles / the previous echo's pass-2 reads
       if (w < N2) two_level_pass1<LC>(v, buf, T::ld, ch, w, tw);
       __syncthreads();
-      if (w < N1 && live) {
+      if (w < N1) {
         float2 zz[N2];
         two_level_pass2<LC>(zz, buf, T::ld, ch, w);
-        float2* dst = a.grid + (((int64_t)zp * gy + yp) * gx) * a.K + (int64_t)e * a.C;
+        // the grid row as a raw buffer: one 32-bit lane offset and the position row k2 a scalar
+        // offset, instead of 16 64-bit addresses; lanes past the chunk's coils store nothing
+        const __amdgpu_buffer_rsrc_t rs =
+            buf_rsrc(reinterpret_cast<char*>(a.grid + (((int64_t)zp * gy + yp) * gx) * a.K + (int64_t)e * a.C));
+        const unsigned lo = live ? ((unsigned)w * (unsigned)a.K + (unsigned)c) * 8u : kBufOob;
+        const unsigned rowb = (unsigned)N1 * (unsigned)a.K * 8u;
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) dst[(int64_t)(w + N1 * k2) * a.K + c] = zz[k2];
+        for (int k2 = 0; k2 < N2; ++k2)
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, zz[k2]),
+                                                rs, lo, (int)(k2 * rowb), 0);
       }
     }
     return;
