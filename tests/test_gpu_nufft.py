@@ -413,3 +413,55 @@ def test_many_echoes_and_long_rows(torch_mod):
         _lib.call("tr_nufft_adjoint", be.plan.handle, 0, qt.data_ptr(), None, None, E, C, g.data_ptr(), 0, s)
         refa = np.stack([on.nufft_adjoint(q[e].astype(np.complex128), coords[0], shape) for e in range(E)])
         assert _rel(g.cpu().numpy(), refa) < TOL
+
+
+@pytest.mark.parametrize("E,C", [(2, 65), (3, 50), (6, 32), (4, 75), (2, 193)])
+def test_interp_channel_layouts_bitwise(torch_mod, E, C):
+    """Channel counts of 130, 150, 192, 300 and 386 (two to four channel groups of the 16-byte-load
+    kernels, the last one ragged: lanes past the group's channels repeat its first pair and store
+    nothing) through every channel-group mode of the interpolation and every epilogue: the forward
+    NUFFT (sample order restored) against the oracle, and the forward NUFFT, the k-space simulation,
+    the residual and the data-dual update on sorted samples bit for bit against the per-sample
+    kernel (mode 0; past its 256-channel limit, the 64-channel-group kernel, mode 1)."""
+    import torch
+
+    from toporecon_hip import _lib
+    from toporecon_hip.nufft_backend import HipNufftBackend
+    K = E * C
+    print("interp channel layouts", E, C, K, flush=True)
+    shape = (8, 12, 10)
+    M = 900
+    rng = np.random.default_rng(K)
+    coords = [_coords(rng, shape, M)]
+    be = HipNufftBackend(shape, coords, 0)
+    dev = torch.device("cuda", 0)
+    img = _cplx(rng, (E,) + shape)
+    maps = _cplx(rng, (C,) + shape)
+    img_t = torch.from_numpy(img).to(dev)
+    maps_t = torch.from_numpy(maps).to(dev)
+    y = torch.from_numpy(_cplx(rng, (M, K))).to(dev)
+    q0 = torch.from_numpy(_cplx(rng, (M, K))).to(dev)
+    s = _lib.stream_handle(dev)
+    h = be.plan.handle
+    modes = (0, 1, 2, 3) if K <= 256 else (1, 2, 3)
+    runs = {}
+    for kg in modes:
+        be.plan.set_param("interp_kgroups", kg)
+        fwd = be.forward_batch(img_t, maps_t, 0).cpu().numpy()
+        sim = torch.empty((M, K), dtype=torch.complex64, device=dev)
+        _lib.call("tr_data_simulate", h, 0, img_t.data_ptr(), maps_t.data_ptr(), E, C, sim.data_ptr(), s)
+        res = torch.empty((M, K), dtype=torch.complex64, device=dev)
+        _lib.call("tr_data_residual", h, 0, img_t.data_ptr(), maps_t.data_ptr(), E, C, y.data_ptr(),
+                  res.data_ptr(), s)
+        q = q0.clone()
+        _lib.call("tr_data_dual_update", h, 0, img_t.data_ptr(), maps_t.data_ptr(), E, C, y.data_ptr(),
+                  q.data_ptr(), 0.25, 1.25, s)
+        torch.cuda.synchronize()
+        runs[kg] = [fwd, sim.cpu().numpy(), res.cpu().numpy(), q.cpu().numpy()]
+    ref = on.nufft(img[:, None].astype(np.complex128) * maps[None], coords[0]).reshape(K, M)
+    assert _rel(runs[3][0].reshape(K, M), ref) < TOL
+    base = runs[modes[0]]
+    assert np.array_equal(base[2], base[1] - y.cpu().numpy())
+    for kg in modes[1:]:
+        for name, a, b in zip(("forward", "simulate", "residual", "dual"), runs[kg], base):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (kg, name)
