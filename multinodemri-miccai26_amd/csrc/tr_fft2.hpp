@@ -127,6 +127,23 @@ constexpr bool two_level_ok() {
   return KC == 16 && LC > 0 && split_n1(LC) > 0 && kBlock / KC == 16;
 }
 
+// Threads of a two-level line workgroup: KC x max(N1, N2) workers rounded up to whole waves, so a
+// length whose split needs fewer than 16 workers per channel (100 = 10 x 10: 160 of 256 threads)
+// leaves no wave idle and more workgroups fit a CU's wave slots.  Other paths use kBlock.
+#ifndef TR_LINES_TRIM
+#define TR_LINES_TRIM 1
+#endif
+template <int KC, int LC>
+constexpr int two_level_threads() {
+  if constexpr (!two_level_ok<KC, LC>() || !TR_LINES_TRIM) {
+    return kBlock;
+  } else {
+    constexpr int n1 = split_n1(LC), n2 = LC / n1;
+    constexpr int t = KC * (n1 > n2 ? n1 : n2);
+    return (t + 63) / 64 * 64;
+  }
+}
+
 // Pass 1 of the two-level FFT for worker j < N2: v = x[j + N2 r] on entry;
 // DFT, twiddle w_L^{j k1} (tw = the line's table, tw[i] = w_L^i) and write
 // the tile row k1 at buf[(k1 N2 + j) ld + ch].

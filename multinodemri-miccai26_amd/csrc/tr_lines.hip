@@ -116,7 +116,8 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
     if (lw.y == 0) {  // empty line: nothing live to load; its outputs are zero
       if (a.zero_empty && k0 + ch < a.K) {
         float2* b0 = a.data + ob * a.outer_stride + in * a.inner_stride + k0 + ch;
-        for (int sidx = w; sidx < a.store_n; sidx += T::W) {
+        const int nw = (int)blockDim.x / KC;  // workers per channel of this launch
+        for (int sidx = w; sidx < a.store_n; sidx += nw) {
           const int p = a.store_n < L ? pmod(sidx - a.store_n / 2, L) : sidx;
           b0[p * a.pos_stride] = make_float2(0.f, 0.f);
         }
@@ -127,7 +128,9 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
   auto live_pos = [&](int p) { return !a.line_win || in_window(p, lw, L); };
   const bool live = k0 + ch < a.K;
   float2* base = a.data + ob * a.outer_stride + in * a.inner_stride + k0 + ch;
-  for (int i = tid; i < LF; i += kBlock) tw[i] = a.tw[i];
+  for (int i = tid; i < LF; i += (int)blockDim.x) tw[i] = a.tw[i];
+  // the two-level path may run with fewer than kBlock threads (two_level_threads); the launch
+  // gives the LDS Stockham path below all kBlock of them (it is taken only without off32)
   if constexpr (two_level_ok<KC, LC>()) {
     if (a.off32) {
       lines_two_level<KC, LC>(a, tw, buf, ch, w, live, lw,
@@ -176,7 +179,8 @@ __global__ void __launch_bounds__(kBlock) k_fft_lines(LinesArgs a) {
 struct LinesLaunch {
   template <int KC, int LC>
   static void go(dim3 g, size_t sm, hipStream_t s, const LinesArgs& a) {
-    hipLaunchKernelGGL((k_fft_lines<KC, LC>), g, dim3(kBlock), sm, s, a);
+    const int threads = a.off32 ? two_level_threads<KC, LC>() : kBlock;
+    hipLaunchKernelGGL((k_fft_lines<KC, LC>), g, dim3(threads), sm, s, a);
   }
 };
 
