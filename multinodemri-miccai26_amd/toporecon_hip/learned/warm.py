@@ -145,6 +145,24 @@ def _normal(solver, x, out, qtmp):
                   solver.maps.data_ptr(), solver.E, solver.C, out[b].data_ptr(), 0, s)
 
 
+def _forward_weighted_sq(solver, x, qtmp, work, sums):
+    """qtmp[b] = A x per bin; returns <A x, W A x> = sum over bins, channels and samples of
+    dcf |A x|^2 (float64 per channel on the device, added in order on the host).  It equals
+    <x, A^H W A x> without the adjoint NUFFT."""
+    s = solver.stream
+    K = solver.E * solver.C
+    for b in range(solver.B):
+        M = int(solver.y[b].shape[0])
+        _lib.call("tr_data_simulate", solver.plan.handle, b, x[b].data_ptr(), solver.maps.data_ptr(), solver.E,
+                  solver.C, qtmp[b].data_ptr(), s)
+        if M == 0:
+            sums[b].zero_()
+            continue
+        _lib.call("tr_data_fit_sums", qtmp[b].data_ptr(), qtmp[b].data_ptr(), solver.dcf[b].data_ptr(), M, K,
+                  work.data_ptr(), FIT_BLOCKS, sums[b].data_ptr(), s)
+    return float(sums.cpu().numpy()[:, :, 1].sum())
+
+
 def _data_adjoint_all(solver, q, out):
     """out = A^H W q over all bins, q per bin in sorted order."""
     s = solver.stream
@@ -160,7 +178,9 @@ def cg_refine(solver, x0, steps: int, rhs=None, x0_zero: bool = False, ax0=None,
     known to be zero, so the initial residual is the rhs itself; ax0: A x0 per bin (sorted k-space)
     if the caller has it, so the initial residual needs only the adjoint.  track_ax: keep A x in
     k-space along the steps (A x += alpha A p, with the A p each step computes anyway) and return
-    (x, A x per bin) -- ax0 (updated in place) or zeros when x0 is zero."""
+    (x, A x per bin) -- ax0 (updated in place) or zeros when x0 is zero.  The last step runs the
+    forward NUFFT only: its step length needs <A p, W A p>, and its residual update (the adjoint) is
+    never used."""
     import torch
     if steps <= 0 and not track_ax:
         return x0
@@ -191,20 +211,34 @@ def cg_refine(solver, x0, steps: int, rhs=None, x0_zero: bool = False, ax0=None,
                   partial.data_ptr(), CG_BLOCKS, s)
     rr = _host_sum(partial)
     _lib.call("tr_cscale", r.data_ptr(), n, 1.0, p.data_ptr(), s)
-    for _ in range(steps):
+    K = solver.E * solver.C
+    fit_work = torch.empty((FIT_BLOCKS, K, 2), dtype=torch.float64, device=dev)
+    fit_sums = torch.empty((solver.B, K, 2), dtype=torch.float64, device=dev)
+    for i in range(steps):
         if rr <= 0.0:
             break
-        _normal(solver, p, ap, qtmp)
-        _lib.call("tr_cdot_re", p.data_ptr(), ap.data_ptr(), n, partial.data_ptr(), CG_BLOCKS, s)
-        pap = _host_sum(partial)
+        last = i == steps - 1
+        if last:
+            # the last step needs only alpha, and <p, A^H W A p> = <A p, W A p> is summed in k-space:
+            # the adjoint NUFFT, which would only update the residual, is not run
+            pap = _forward_weighted_sq(solver, p, qtmp, fit_work, fit_sums)
+        else:
+            _normal(solver, p, ap, qtmp)
+            _lib.call("tr_cdot_re", p.data_ptr(), ap.data_ptr(), n, partial.data_ptr(), CG_BLOCKS, s)
+            pap = _host_sum(partial)
         if not pap > 0.0:
             break
         alpha = rr / pap
-        _lib.call("tr_cg_update", x.data_ptr(), r.data_ptr(), p.data_ptr(), ap.data_ptr(), n, alpha,
-                  partial.data_ptr(), CG_BLOCKS, s)
+        if last:
+            _lib.call("tr_js_axpy", x.data_ptr(), p.data_ptr(), n, alpha, s)  # x += alpha p
+        else:
+            _lib.call("tr_cg_update", x.data_ptr(), r.data_ptr(), p.data_ptr(), ap.data_ptr(), n, alpha,
+                      partial.data_ptr(), CG_BLOCKS, s)
         if ax is not None:  # A x += alpha A p (qtmp holds A p of this step)
             for b in range(solver.B):
                 _lib.call("tr_js_axpy", ax[b].data_ptr(), qtmp[b].data_ptr(), qtmp[b].numel(), alpha, s)
+        if last:
+            break
         rr_new = _host_sum(partial)
         _lib.call("tr_cg_direction", p.data_ptr(), r.data_ptr(), n, rr_new / rr, s)
         rr = rr_new
