@@ -258,8 +258,25 @@ __global__ void __launch_bounds__(256) k_interp_grp(InterpArgs a, int G) {
 #pragma unroll
   for (int t = 0; t < kTapGroup; ++t) acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
   constexpr int RB = kInterpRows;
+  // The members' y and x weights are read once per group into scalar registers and the z weights
+  // once per plane, so no row waits for a scalar load (reading them from the records inside the
+  // row loop put a scalar load and its wait in front of every member's row: 2.71 -> 2.62 ms).
+  // Members past the group size read the last member's record and are never used.
+  float wyv[kTapGroup][5], wxv[kTapGroup][5];
+#pragma unroll
+  for (int t = 0; t < kTapGroup; ++t) {
+    const TapRec& r = rec[min(t, gs - 1)];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      wyv[t][j] = r.w[5 + j];
+      wxv[t][j] = r.w[10 + j];
+    }
+  }
   for (int dz = 0; dz < nz; ++dz) {
     const int pz = wrap2(p0z + dz, gz);
+    float wzv[kTapGroup];
+#pragma unroll
+    for (int t = 0; t < kTapGroup; ++t) wzv[t] = rec[min(t, gs - 1)].w[dz];
 #pragma unroll
     for (int d0 = 0; d0 < 4; d0 += RB) {
       float4 v[RB][4];
@@ -281,11 +298,10 @@ __global__ void __launch_bounds__(256) k_interp_grp(InterpArgs a, int G) {
 #pragma unroll
         for (int t = 0; t < kTapGroup; ++t) {
           if (t < gs) {
-            const TapRec& r = rec[t];
-            const float wzy = r.w[dz] * r.w[5 + dy];
+            const float wzy = wzv[t] * wyv[t][dy];
 #pragma unroll
-            for (int dx = 0; dx < 4; ++dx) acc4(acc[t], wzy * r.w[10 + dx], u[dx]);
-            if (nx == 5) acc4(acc[t], wzy * r.w[14], u[4]);
+            for (int dx = 0; dx < 4; ++dx) acc4(acc[t], wzy * wxv[t][dx], u[dx]);
+            if (nx == 5) acc4(acc[t], wzy * wxv[t][4], u[4]);
           }
         }
       }
