@@ -68,7 +68,12 @@ __device__ __forceinline__ void stage_patch(uint16_t* patch, const uint16_t* x, 
                                             int X) {
   constexpr int NCB = CIN / 8, PX = kCnnTX + 2, PY = kCnnTY + 2, PZ = 3;
   const uint16_t* src = x + (int64_t)t.img * Z * Y * X * CIN;
-  for (int c = threadIdx.x; c < PZ * PY * PX * NCB; c += blockDim.x) {
+  // every load of the thread's share is issued before any is written to LDS (a 256-thread block)
+  constexpr int NTOT = PZ * PY * PX * NCB, NPER = (NTOT + 255) / 256;
+  uint4 val[NPER];
+#pragma unroll
+  for (int i = 0; i < NPER; ++i) {
+    const int c = (int)threadIdx.x + i * 256;
     const int cb = c % NCB;
     int v = c / NCB;
     const int px = v % PX;
@@ -76,12 +81,24 @@ __device__ __forceinline__ void stage_patch(uint16_t* patch, const uint16_t* x, 
     const int py = v % PY;
     const int pz = v / PY;
     const int gz = t.z + pz - 1, gy = t.y0 + py - 1, gx = t.x0 + px - 1;
-    uint4 val = make_uint4(0u, 0u, 0u, 0u);
-    if (gz >= 0 && gz < Z && gy >= 0 && gy < Y && gx >= 0 && gx < X)
-      val = *(const uint4*)(src + (((int64_t)gz * Y + gy) * X + gx) * CIN + cb * 8);
-    uint2* dst = (uint2*)(patch + ((pz * PY + py) * PX + px) * VS + cb * 8);
-    dst[0] = make_uint2(val.x, val.y);
-    dst[1] = make_uint2(val.z, val.w);
+    val[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (c < NTOT && gz >= 0 && gz < Z && gy >= 0 && gy < Y && gx >= 0 && gx < X)
+      val[i] = *(const uint4*)(src + (((int64_t)gz * Y + gy) * X + gx) * CIN + cb * 8);
+  }
+#pragma unroll
+  for (int i = 0; i < NPER; ++i) {
+    const int c = (int)threadIdx.x + i * 256;
+    if (c < NTOT) {
+      const int cb = c % NCB;
+      int v = c / NCB;
+      const int px = v % PX;
+      v /= PX;
+      const int py = v % PY;
+      const int pz = v / PY;
+      uint2* dst = (uint2*)(patch + ((pz * PY + py) * PX + px) * VS + cb * 8);
+      dst[0] = make_uint2(val[i].x, val[i].y);
+      dst[1] = make_uint2(val[i].z, val[i].w);
+    }
   }
 }
 
