@@ -54,9 +54,22 @@ __global__ void __launch_bounds__(kBlock, (fwd_x_min_blocks<KC, LC>())) k_fwd_x_
     float2* img_row = buf + LC * T::ld;
     auto stage_rows = [&](int e0) {
       const int ne = min(EG, E - e0);
-      for (int i = tid; i < ne * nx; i += kBlock) {
-        const int e = i / nx;
-        img_row[i] = a.img[(int64_t)(e0 + e) * N + rowvox + (i - e * nx)];
+      // 4 loads per thread in flight before their LDS writes (1.27 -> 1.25 ms per launch)
+      const int n = ne * nx;
+      for (int i0 = tid; i0 < n; i0 += 4 * kBlock) {
+        float2 t4[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int i = i0 + k * kBlock;
+          const int e = i / nx;
+          t4[k] = make_float2(0.f, 0.f);
+          if (i < n) t4[k] = a.img[(int64_t)(e0 + e) * N + rowvox + (i - e * nx)];  // only i < n is read
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int i = i0 + k * kBlock;
+          if (i < n) img_row[i] = t4[k];
+        }
       }
     };
     stage_rows(0);
