@@ -49,6 +49,19 @@ const char* last_error();
 // of several iterations leaves an instance's state untouched once its own
 // stopping rule has closed the gate.  Null = no gate.
 // ---------------------------------------------------------------------------
+// Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// With chunk S > 0 the logical block order is permuted so that every XCD
+// receives runs of S consecutive logical blocks, which then share its L2; the
+// tail that does not fill whole 8 * S groups keeps the identity order.
+constexpr unsigned kNumXcd = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned S) {
+  if (S == 0) return b;
+  const unsigned full = nb / (kNumXcd * S) * (kNumXcd * S);
+  if (b >= full) return b;
+  const unsigned r = b / kNumXcd, x = b % kNumXcd;
+  return (r / S) * (kNumXcd * S) + x * S + (r % S);
+}
+
 const int* gate_current();
 __device__ __forceinline__ bool gate_closed(const int* g) { return g != nullptr && *g != 0; }
 
