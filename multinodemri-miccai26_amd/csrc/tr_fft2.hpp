@@ -130,12 +130,9 @@ constexpr bool two_level_ok() {
 // Threads of a two-level line workgroup: KC x max(N1, N2) workers rounded up to whole waves, so a
 // length whose split needs fewer than 16 workers per channel (100 = 10 x 10: 160 of 256 threads)
 // leaves no wave idle and more workgroups fit a CU's wave slots.  Other paths use kBlock.
-#ifndef TR_LINES_TRIM
-#define TR_LINES_TRIM 1
-#endif
 template <int KC, int LC>
 constexpr int two_level_threads() {
-  if constexpr (!two_level_ok<KC, LC>() || !TR_LINES_TRIM) {
+  if constexpr (!two_level_ok<KC, LC>()) {
     return kBlock;
   } else {
     constexpr int n1 = split_n1(LC), n2 = LC / n1;
