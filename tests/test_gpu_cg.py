@@ -71,12 +71,15 @@ def _oracle_cg(prob, x0, steps):
     return x, data
 
 
-@pytest.mark.parametrize("start", ["zeros", "random"])
-def test_cg_refine_matches_float64_cg(torch_mod, start):
+@pytest.mark.parametrize("start,steps", [("zeros", 4), ("random", 4), ("random", 1)])
+def test_cg_refine_matches_float64_cg(torch_mod, start, steps):
+    """cg_refine against Hestenes-Stiefel CG in float64 over the oracle's operators; its last step
+    takes the step length from <A p, W A p> in k-space and runs no adjoint, so steps = 1 is that
+    path alone."""
     torch = torch_mod
     from toporecon_hip.algorithms import TvmeReconstructor
     from toporecon_hip.learned import cg_refine
-    print("cg refine", start, flush=True)
+    print("cg refine", start, steps, flush=True)
     acq, ksp, prob = make_problem(shape=(16, 24, 24), coils=3, echoes=2, bins=2, interleaves=120,
                                   readouts=40, seed=8, algorithm="tvme")
     B = len(acq.bin_edges) - 1
@@ -88,7 +91,6 @@ def test_cg_refine_matches_float64_cg(torch_mod, start):
     rng = np.random.default_rng(5)
     x0n = np.zeros(dims, np.complex64) if start == "zeros" else 0.1 * _cvec(rng, int(np.prod(dims))).reshape(dims)
     x0 = torch.from_numpy(x0n).cuda()
-    steps = 4
     x_gpu = cg_refine(s, x0, steps).cpu().numpy()
     assert np.array_equal(x0.cpu().numpy(), x0n)  # the start is not modified
     x_ref, data = _oracle_cg(prob, x0n, steps)
@@ -149,3 +151,4 @@ def test_warm_start_dual_from_tracked_kspace(torch_mod, with_net):
     assert rel(a, r) < 1e-5
     with pytest.raises(ValueError):
         s.run(init=u, dual_init=q0[:1])
+
