@@ -182,7 +182,7 @@ def test_default_settings(torch_mod):
     finally:
         s.close()
     dh = max(abs(a - b) / b for a, b in zip(h, h_ref))
-    print(f"round 1: objective {dh:.2e}, map kernels {e:.2e}", flush=True)
+    print(f"first alternation: objective {dh:.2e}, map kernels {e:.2e}", flush=True)
     assert dh < 1e-4 and e < 1e-4
     # the full default run
     h = []
