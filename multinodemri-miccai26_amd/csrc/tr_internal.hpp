@@ -261,8 +261,8 @@ enum InterpMode : int { kInterpPlain = 0, kInterpDual = 1, kInterpSim = 2, kInte
 
 constexpr int kTapGroup = 4;  // samples sharing one tap gather in the grouped interpolation
 // y rows whose 4 x taps the grouped interpolation loads together (1, 2 or 4): one row keeps the kernel
-// at 51 VGPRs (8 waves per SIMD), which hides the gathers' latency better than 16 loads in flight per
-// wave at 94 VGPRs (5 waves): 2.80 -> 2.71 ms per launch at 192 channels
+// at 51 VGPRs, which hides the gathers' latency better than 16 loads in flight per wave at 94 VGPRs
+// (5 waves per SIMD): 2.80 -> 2.71 ms per launch at 192 channels
 #ifndef TR_INTERP_ROWS
 #define TR_INTERP_ROWS 1
 #endif
